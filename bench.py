@@ -1,0 +1,231 @@
+"""Benchmark of the point_vortex_hip velocity backend on one MI355X.
+
+Workload (a "step" is one classical RK4 step = four velocity evaluations with the fused stage
+arithmetic): disk R=1, N=65536 point vortices, 1% of them at r in [0.995, 0.9999], circulations
+N(0,1)/N, seed 2, dt=1e-4, 200 steps -- the N=64K configuration the headline metric is quoted
+on.  Inputs are resident in HBM before the timed region; the timed region is K steps enqueued
+back to back with one synchronisation at the end.
+
+Printed: one JSON line.  `value` is FP64 pair-interactions/s (the reference's convention,
+N(N-1) per evaluation, src/benchmark.cpp:43).  The `mixed` object holds the mixed-precision run
+and its normwise velocity error against the FP64 kernel on the same states.  `roofline` is
+the FP64 disk pair kernel's algorithmic FLOP rate (36 flops per pair) over its average launch
+duration from HIP events on the handle's stream.  `cpu_baseline` is the reference's own CPU
+kernel (oracle/_ref, built from the reference sources) or, if absent, the repo's C
+restatement, timed on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("pair-interactions/s at N=64K (fp64 vs mixed @1e-6 rel); batched solves/s + p99")
+FLOPS_PER_PAIR = {"infinite": 12, "disk": 36, "periodic": 12 + 22 * 17}
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def normwise(u, v, ur, vr):
+    return float(np.sqrt(np.sum((u - ur) ** 2 + (v - vr) ** 2) / np.sum(ur ** 2 + vr ** 2)))
+
+
+def run_rk4(pv, pop, precision, steps, warmup, dt):
+    k = pv.HipKernel(pv.SimParams(boundaryCondition="disk", diskRadius=1.0, precision=precision))
+    s = pv.VortexSystem(pop.x, pop.y, pop.circulation)
+    k.uploadDeviceState(s)
+    if warmup:
+        k.deviceRk4Advance(dt, warmup)
+    k.uploadDeviceState(s)  # the timed run starts from the configured initial state
+    k.profileBegin()
+    t0 = time.perf_counter()
+    k.deviceRk4Advance(dt, steps)
+    t1 = time.perf_counter()
+    kernel_ms, launches = k.profileEnd()
+    return k, t1 - t0, kernel_ms, launches
+
+
+def per_geometry(pv, n, repeats=3):
+    """Evaluation-only pair-interactions/s for the three geometries at N (FP64 and mixed)."""
+    from pointvortex_amd import workloads as wl
+
+    out = {}
+    pops = {"infinite": wl.infinite(n, 0), "periodic": wl.periodic(n, 1),
+            "disk": wl.disk(n, 2)}
+    for geo, pop in pops.items():
+        row = {}
+        ref_vel = None
+        for prec in ("fp64", "mixed"):
+            k = pv.HipKernel(pv.SimParams(boundaryCondition=geo, boxLengthX=2 * np.pi,
+                                          boxLengthY=2 * np.pi, precision=prec))
+            k.uploadDeviceState(pv.VortexSystem(pop.x, pop.y, pop.circulation))
+            k.evaluateResident(1)
+            k.synchronize()
+            k.profileBegin()
+            t0 = time.perf_counter()
+            k.evaluateResident(repeats)
+            k.synchronize()
+            dt = time.perf_counter() - t0
+            ms, launches = k.profileEnd()
+            pairs = repeats * n * (n - 1)
+            vel = k.evaluateDeviceState()
+            if prec == "fp64":
+                ref_vel = vel
+            peak = PEAK_FP64_TFLOPS if prec == "fp64" else PEAK_FP32_TFLOPS
+            kt = ms / 1e3
+            row[prec] = {
+                "pairs_per_s": pairs / dt,
+                "kernel_pairs_per_s": pairs / kt,
+                "kernel_ms_per_eval": ms / launches,
+                "algorithmic_tflops": pairs * FLOPS_PER_PAIR[geo] / kt / 1e12,
+                "frac_of_peak": pairs * FLOPS_PER_PAIR[geo] / kt / 1e12 / peak,
+            }
+            if prec == "mixed":
+                row[prec]["normwise_err_vs_fp64"] = normwise(vel.x, vel.y, ref_vel.x, ref_vel.y)
+            k.close()
+        row["mixed_speedup"] = row["mixed"]["kernel_pairs_per_s"] / row["fp64"]["kernel_pairs_per_s"]
+        out[geo] = row
+        log(f"[bench] per-geometry {geo}: fp64 {row['fp64']['kernel_pairs_per_s']:.3e} "
+            f"mixed {row['mixed']['kernel_pairs_per_s']:.3e} pairs/s")
+    return out
+
+
+def cpu_baseline(pop, budget_s=20.0):
+    """Reference CPU disk kernel (src/compute.cpp:286-336) on this host, bounded sample: as many
+    full N=65536 evaluations as fit in about budget_s (at least one)."""
+    from oracle import pyoracle
+
+    kind = "reference" if pyoracle.Reference.available() else "port"
+    impl = pyoracle.Reference() if kind == "reference" else pyoracle.Oracle()
+    k = pyoracle.spec("disk", radius=1.0)
+    n = len(pop.x)
+    t0 = time.perf_counter()
+    impl.evaluate(k, pop.x, pop.y, pop.circulation)
+    first = time.perf_counter() - t0
+    reps = max(0, min(4, int(budget_s / max(first, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        impl.evaluate(k, pop.x, pop.y, pop.circulation)
+    total = time.perf_counter() - t0
+    evals, secs = (reps, total) if reps else (1, first)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": evals * n * (n - 1) / secs,
+        "unit": "pair-interactions/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"{evals} FP64 disk evaluation(s) of the N={n} workload state "
+                  f"({secs:.1f} s, OpenMP {threads} threads, host nproc {os.cpu_count()})",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the per-geometry table")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this backend has no multi-GPU path "
+              "(each GPU would run its own share; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import pointvortex_amd as pv
+    from pointvortex_amd import workloads as wl
+
+    dt = 1e-4
+    n = args.n
+    pop = wl.disk(n, 2)
+    pairs_per_step = 4 * n * (n - 1)
+    log(f"[bench] disk N={n} RK4 dt={dt} steps={args.steps} warmup={args.warmup}")
+
+    k64, wall64, kms64, launches64 = run_rk4(pv, pop, "fp64", args.steps, args.warmup, dt)
+    value = args.steps * pairs_per_step / wall64
+    log(f"[bench] fp64: {wall64:.3f} s, {value:.4e} pairs/s, kernel {kms64 / launches64:.3f} ms")
+    kmx, wallmx, kmsmx, launchesmx = run_rk4(pv, pop, "mixed", args.steps, args.warmup, dt)
+    value_mx = args.steps * pairs_per_step / wallmx
+    log(f"[bench] mixed: {wallmx:.3f} s, {value_mx:.4e} pairs/s")
+
+    # Velocity error of the mixed kernel against the FP64 kernel on the mixed run's final state,
+    # and on the initial state.
+    errs = []
+    fp64_eval = pv.HipKernel(pv.SimParams(boundaryCondition="disk", diskRadius=1.0))
+    for state in ("initial", "final"):
+        s = pv.VortexSystem(pop.x.copy(), pop.y.copy(), pop.circulation)
+        if state == "final":
+            kmx.downloadDeviceState(s)
+        kmx.uploadDeviceState(s) if state == "initial" else None
+        fp64_eval.uploadDeviceState(s)
+        ref = fp64_eval.evaluateDeviceState()
+        got = kmx.evaluateDeviceState()
+        errs.append(normwise(got.x, got.y, ref.x, ref.y))
+    log(f"[bench] mixed normwise error vs fp64: {errs}")
+
+    avg_ms = kms64 / launches64
+    achieved = n * (n - 1) * FLOPS_PER_PAIR["disk"] / (avg_ms / 1e3) / 1e12
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall64 / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"disk R=1, N={n} (1% at r in [0.995,0.9999]), G~N(0,1)/N, seed 2, "
+                        f"RK4 dt=1e-4, {args.steps} steps",
+            "geometry": "disk",
+            "n": n,
+            "integrator": "rk4",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel": "pair_kernel<DiskF64>",
+            "flops_per_pair": FLOPS_PER_PAIR["disk"],
+            "avg_launch_ms": avg_ms,
+            "launches": launches64,
+        },
+        "mixed": {
+            "value": value_mx,
+            "unit": "pair-interactions/s",
+            "dtype": "f64-differences/f32-terms/f64-accumulation",
+            "speedup_vs_fp64": value_mx / value,
+            "normwise_err_vs_fp64_initial": errs[0],
+            "normwise_err_vs_fp64_final": errs[1],
+            "avg_launch_ms": kmsmx / launchesmx,
+        },
+    }
+    if not args.no_extra:
+        result["per_geometry_n65536"] = per_geometry(pv, n)
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        result["cpu_baseline"] = cpu_baseline(pop)
+        log(f"[bench] CPU: {result['cpu_baseline']['value']:.3e} pairs/s")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * pv_hip.h -- C-ABI of point_vortex_hip, the MI355X (gfx950) velocity backend for
+ * PointVortex.
+ *
+ * Plain C types only: pointers to host double arrays, sizes, and small POD structs.
+ * Every call returns a pv_status; on failure pv_last_error() holds the message the
+ * reference would have thrown (same text where the reference has one).
+ *
+ * Status codes map one-to-one onto the reference's exception types:
+ *   PV_ERR_INVALID_ARGUMENT -> std::invalid_argument
+ *   PV_ERR_OUT_OF_RANGE     -> std::out_of_range
+ *   PV_ERR_RUNTIME          -> std::runtime_error   (includes HIP API failures, reported
+ *                                                     as "<operation>: <hipGetErrorString>")
+ *   PV_ERR_LOGIC            -> std::logic_error
+ *
+ * A handle is used by one host thread at a time (as the reference's VelocityKernel
+ * objects are); different handles may be used concurrently.  Each handle owns its
+ * device buffers and one HIP stream.
+ */
+#ifndef PV_HIP_H
+#define PV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pv_status {
+    PV_OK = 0,
+    PV_ERR_INVALID_ARGUMENT = 1,
+    PV_ERR_OUT_OF_RANGE = 2,
+    PV_ERR_RUNTIME = 3,
+    PV_ERR_LOGIC = 4
+} pv_status;
+
+/* Boundary conditions of src/params.h:52 (boundaryCondition). */
+typedef enum pv_geometry { PV_INFINITE = 0, PV_PERIODIC = 1, PV_DISK = 2 } pv_geometry;
+
+/* Arithmetic of the pair sums.  PV_FP64 is the drop-in (matches the reference's FP64 CPU
+ * path to rounding).  PV_MIXED forms pair differences in FP64, evaluates the pair terms in
+ * FP32 and folds per-tile FP32 partial sums into FP64 with compensated (TwoSum)
+ * accumulation; it has no counterpart in the reference. */
+typedef enum pv_precision { PV_FP64 = 0, PV_MIXED = 1 } pv_precision;
+
+/* Kernel parameters: the geometry fields of SimParams (src/params.h:44-57) that
+ * makeReferenceKernel (src/backend_common.cpp:4-13) reads. */
+typedef struct pv_config {
+    int geometry;          /* pv_geometry */
+    int precision;         /* pv_precision */
+    double core_radius;    /* coreRadius, infinite plane only */
+    double box_length_x;   /* boxLengthX, periodic only */
+    double box_length_y;   /* boxLengthY, periodic only (must equal box_length_x) */
+    int image_layers;      /* periodicImageLayers, 0..64 */
+    double disk_radius;    /* diskRadius, disk only */
+    int device;            /* HIP device ordinal */
+} pv_config;
+
+/* StepResult (src/compute.h:11-16). */
+typedef struct pv_step_result {
+    double accepted_time_step;
+    double suggested_time_step;
+    double normalized_error;
+    unsigned rejected_steps;
+} pv_step_result;
+
+/* Invariants (src/compute.h:78-81). */
+typedef struct pv_invariants {
+    double circulation;
+    double linear_impulse_x;
+    double linear_impulse_y;
+    double angular_impulse;
+    double hamiltonian;
+} pv_invariants;
+
+typedef struct pv_handle pv_handle;
+
+/* Message of the last failed call on this thread (never NULL). */
+const char *pv_last_error(void);
+
+/* Replaces makeBackendKernel(params) (src/backend.h:16, src/backend_cuda.cu:550-552): the
+ * constructor checks of InfinitePlaneKernel/PeriodicBoxKernel/DiskKernel
+ * (src/compute.cpp:50-56, 165-173, 258-261) with the same messages. */
+pv_status pv_create(const pv_config *config, pv_handle **out);
+void pv_destroy(pv_handle *h);
+
+/* Switch the pair-sum arithmetic of an existing handle (no reference counterpart). */
+pv_status pv_set_precision(pv_handle *h, int precision);
+
+/* VelocityKernel::evaluateRange (src/compute.h:24-26, src/compute.cpp:88-133, 203-256,
+ * 286-336): pure evaluation of host arrays; u[begin..end), v[begin..end) are written,
+ * other entries are left untouched.  Does not disturb the uploaded device state (unlike
+ * src/backend_cuda.cu:229-236). */
+pv_status pv_evaluate_range(pv_handle *h, const double *x, const double *y, const double *gamma,
+                            size_t n, double *u, double *v, size_t begin, size_t end);
+
+/* VelocityKernel::evaluate (src/compute.cpp:60-69): full range plus the non-finite check. */
+pv_status pv_evaluate(pv_handle *h, const double *x, const double *y, const double *gamma,
+                      size_t n, double *u, double *v);
+
+/* VelocityKernel::hamiltonian (src/compute.cpp:85-87, 174-202, 262-285) of host arrays,
+ * computed on the device. */
+pv_status pv_hamiltonian(pv_handle *h, const double *x, const double *y, const double *gamma,
+                         size_t n, double *out);
+
+/* computeInvariants(vortices, kernel) (src/compute.cpp:157-163) of host arrays. */
+pv_status pv_invariants_of(pv_handle *h, const double *x, const double *y, const double *gamma,
+                           size_t n, pv_invariants *out);
+
+/* uploadDeviceState (src/compute.h:30, src/backend_cuda.cu:249-269). */
+pv_status pv_upload(pv_handle *h, const double *x, const double *y, const double *gamma,
+                    size_t n);
+
+/* downloadDeviceState (src/compute.h:31, src/backend_cuda.cu:270-282): n must equal the
+ * uploaded population.  gamma may be NULL. */
+pv_status pv_download(pv_handle *h, double *x, double *y, double *gamma, size_t n);
+
+/* Number of vortices in the device state. */
+pv_status pv_state_size(pv_handle *h, size_t *n);
+
+/* evaluateDeviceState (src/compute.h:32, src/backend_cuda.cu:283-298). */
+pv_status pv_evaluate_device_state(pv_handle *h, double *u, double *v, size_t n);
+
+/* deviceRk4Step (src/compute.h:33, src/backend_cuda.cu:299-323): one classical RK4 step of
+ * the device state, validated before it returns. */
+pv_status pv_rk4_step(pv_handle *h, double dt);
+
+/* Enqueue `steps` RK4 steps of the device state with one synchronisation at the end (no
+ * reference counterpart; the reference synchronises after every evaluation).  On failure
+ * the message names the first failing step; the state is that of the last good step. */
+pv_status pv_rk4_advance(pv_handle *h, double dt, int steps);
+
+/* deviceDopri5Step (src/compute.h:34, src/backend_cuda.cu:324-357, src/timestep.cpp:62-119):
+ * FSAL, max-norm error estimate on the device, the reference's controller and rejection
+ * rule. */
+pv_status pv_dopri5_step(pv_handle *h, double dt, double atol, double rtol, double min_dt,
+                         double max_dt, pv_step_result *out);
+
+/* invalidateDeviceDerivative (src/compute.h:35, src/backend_cuda.cu:358). */
+void pv_invalidate_fsal(pv_handle *h);
+
+/* computeInvariants of the device state (moments and the FP64 Hamiltonian log-sum on the
+ * device; replaces the CPU Hamiltonian of src/backend_cuda.cu:245-247). */
+pv_status pv_state_invariants(pv_handle *h, pv_invariants *out);
+
+/* The O(N^2) search of DipoleManager::process (src/dipole.cpp:87-119) on the device state:
+ * opposite-sign nonzero pairs i<j closer than `distance` (periodic: minimum image), plus in
+ * a disk the real/image wall candidate written as (i, n).  Candidates come back unordered;
+ * the caller sorts them (src/dipole.cpp:120-125).  *count receives the number found, which
+ * may exceed cap (then only cap entries are written). */
+pv_status pv_dipole_candidates(pv_handle *h, double distance, uint64_t *first, uint64_t *second,
+                               double *distance_squared, size_t cap, size_t *count);
+
+/* backendRuntimeDetails (src/backend.h:14): newline-separated "key value" lines. */
+pv_status pv_runtime_details(int device, char *buffer, size_t length);
+
+/* Device-resident velocity evaluation for benchmarks: evaluates the uploaded state into
+ * device buffers and does not copy back or synchronise.  pv_synchronize waits and reports
+ * any sticky failure. */
+pv_status pv_evaluate_resident(pv_handle *h, int repeats);
+pv_status pv_synchronize(pv_handle *h);
+
+/* Launch-shape override for the pair kernels (no reference counterpart): targets per
+ * thread (1, 2 or 4) and number of source splits (0 = automatic).  Used by the autotuner. */
+pv_status pv_set_launch(pv_handle *h, int targets_per_thread, int source_splits);
+
+/* Kernel timing for benchmarks (no reference counterpart): between begin and end every
+ * velocity-kernel launch of the handle is bracketed by HIP events recorded on the handle's own
+ * stream; end waits and returns the summed kernel time and the number of launches. */
+pv_status pv_profile_begin(pv_handle *h);
+pv_status pv_profile_end(pv_handle *h, double *kernel_ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PV_HIP_H */

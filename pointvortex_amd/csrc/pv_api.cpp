@@ -1,0 +1,1054 @@
+// pv_api.cpp -- the C-ABI of point_vortex_hip (include/pv_hip.h).
+//
+// A handle owns its device buffers and one HIP stream.  Velocity evaluations, RK stage
+// arithmetic, the DOPRI5 error norm, invariants and the dipole search all run on the device;
+// the host validates arguments (in the reference's order, with its messages), chooses launch
+// shapes, runs the DOPRI5 step-size controller on the one reduced error value per attempt,
+// and turns the sticky device status into the reference's exceptions.
+#include "pv_hip.h"
+#include "pv_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <memory>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_error;
+
+struct ApiError {
+    pv_status code;
+    std::string message;
+};
+
+[[noreturn]] void raise(pv_status code, const std::string &message) {
+    throw ApiError{code, message};
+}
+
+void hip_check(hipError_t status, const char *operation) {
+    if (status != hipSuccess)
+        raise(PV_ERR_RUNTIME, std::string(operation) + ": " + hipGetErrorString(status));
+}
+
+template <typename F> pv_status guard(F &&body) {
+    try {
+        body();
+        return PV_OK;
+    } catch (const ApiError &e) {
+        g_error = e.message;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_error = "out of host memory";
+        return PV_ERR_RUNTIME;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return PV_ERR_RUNTIME;
+    }
+}
+
+struct DeviceGuard {
+    int previous = 0;
+    bool changed = false;
+    explicit DeviceGuard(int device) {
+        hip_check(hipGetDevice(&previous), "query HIP device");
+        if (previous != device) {
+            hip_check(hipSetDevice(device), "select HIP device");
+            changed = true;
+        }
+    }
+    ~DeviceGuard() {
+        if (changed)
+            (void)hipSetDevice(previous);
+    }
+};
+
+template <typename T> void dfree(T *&p) {
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T> T *dalloc(size_t count, const char *what) {
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), what);
+    return static_cast<T *>(p);
+}
+
+// src/timestep.cpp:104-108 (b5, b4) and src/timestep.h:8-15 (a_{s,j}).
+constexpr double kB5[7] = {35.0 / 384.0,     0.0, 500.0 / 1113.0, 125.0 / 192.0,
+                           -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
+constexpr double kB4[7] = {5179.0 / 57600.0, 0.0,           7571.0 / 16695.0, 393.0 / 640.0,
+                           -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0};
+constexpr double kDopriA[7][7] = {
+    {0},
+    {1.0 / 5.0},
+    {3.0 / 40.0, 9.0 / 40.0},
+    {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0},
+    {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0},
+    {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0},
+    {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+
+// What a launch evaluated, so that a failure can be classified afterwards.
+struct EvalRecord {
+    const double *x, *y;
+    int n, tbegin, tend;
+};
+
+} // namespace
+
+struct pv_handle {
+    pv_config cfg{};
+    pv::GeoParams gp{};
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+
+    size_t n = 0, cap = 0;
+    bool state_valid = false;
+    bool fsal_valid = false;
+    double *x = nullptr, *y = nullptr, *g = nullptr;
+    double *xalt = nullptr, *yalt = nullptr;
+    double *xa = nullptr, *ya = nullptr, *xb = nullptr, *yb = nullptr;
+    double *kx[7] = {}, *ky[7] = {};
+    double *u = nullptr, *v = nullptr;
+
+    size_t scap = 0;
+    double *sx = nullptr, *sy = nullptr, *sg = nullptr, *su = nullptr, *sv = nullptr;
+
+    double *part = nullptr;
+    size_t part_cap = 0;
+    unsigned *tickets = nullptr;
+    size_t tickets_cap = 0;
+    double *red = nullptr;
+    size_t red_cap = 0;
+    unsigned long long *cand_first = nullptr, *cand_second = nullptr;
+    double *cand_d2 = nullptr;
+    size_t cand_cap = 0;
+
+    pv::DevStatus *st = nullptr;
+    pv::DevStatus *st_host = nullptr;
+
+    int tpt_override = 0, split_override = 0;
+    bool profiling = false;
+    std::vector<hipEvent_t> events;   // pool, two per profiled launch
+    size_t events_used = 0;
+    unsigned epoch = 0;
+    unsigned epoch_base = 1;
+    std::vector<EvalRecord> records;
+
+    ~pv_handle() {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        dfree(x), dfree(y), dfree(g), dfree(xalt), dfree(yalt);
+        dfree(xa), dfree(ya), dfree(xb), dfree(yb);
+        for (int s = 0; s < 7; ++s)
+            dfree(kx[s]), dfree(ky[s]);
+        dfree(u), dfree(v);
+        dfree(sx), dfree(sy), dfree(sg), dfree(su), dfree(sv);
+        dfree(part), dfree(tickets), dfree(red);
+        dfree(cand_first), dfree(cand_second), dfree(cand_d2);
+        dfree(st);
+        for (hipEvent_t e : events)
+            (void)hipEventDestroy(e);
+        if (st_host)
+            (void)hipHostFree(st_host);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void check_population(size_t n) {
+    if (n > (size_t)(INT_MAX / 4))
+        raise(PV_ERR_INVALID_ARGUMENT, "vortex population exceeds the supported size");
+}
+
+void ensure_state(pv_handle *h, size_t n) {
+    if (n <= h->cap && h->x)
+        return;
+    hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+    dfree(h->x), dfree(h->y), dfree(h->g), dfree(h->xalt), dfree(h->yalt);
+    dfree(h->xa), dfree(h->ya), dfree(h->xb), dfree(h->yb);
+    for (int s = 0; s < 7; ++s)
+        dfree(h->kx[s]), dfree(h->ky[s]);
+    dfree(h->u), dfree(h->v);
+    h->cap = 0;
+    h->state_valid = false;
+    h->x = dalloc<double>(n, "hipMalloc(x)");
+    h->y = dalloc<double>(n, "hipMalloc(y)");
+    h->g = dalloc<double>(n, "hipMalloc(circulation)");
+    h->xalt = dalloc<double>(n, "hipMalloc(x next)");
+    h->yalt = dalloc<double>(n, "hipMalloc(y next)");
+    h->xa = dalloc<double>(n, "hipMalloc(stage x)");
+    h->ya = dalloc<double>(n, "hipMalloc(stage y)");
+    h->xb = dalloc<double>(n, "hipMalloc(stage x)");
+    h->yb = dalloc<double>(n, "hipMalloc(stage y)");
+    for (int s = 0; s < 7; ++s) {
+        h->kx[s] = dalloc<double>(n, "hipMalloc(x stage)");
+        h->ky[s] = dalloc<double>(n, "hipMalloc(y stage)");
+    }
+    h->u = dalloc<double>(n, "hipMalloc(u)");
+    h->v = dalloc<double>(n, "hipMalloc(v)");
+    h->cap = n;
+}
+
+void ensure_scratch(pv_handle *h, size_t n) {
+    if (n <= h->scap && h->sx)
+        return;
+    hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+    dfree(h->sx), dfree(h->sy), dfree(h->sg), dfree(h->su), dfree(h->sv);
+    h->scap = 0;
+    h->sx = dalloc<double>(n, "hipMalloc(scratch x)");
+    h->sy = dalloc<double>(n, "hipMalloc(scratch y)");
+    h->sg = dalloc<double>(n, "hipMalloc(scratch circulation)");
+    h->su = dalloc<double>(n, "hipMalloc(scratch u)");
+    h->sv = dalloc<double>(n, "hipMalloc(scratch v)");
+    h->scap = n;
+}
+
+void ensure_partials(pv_handle *h, size_t doubles) {
+    if (doubles <= h->part_cap && h->part)
+        return;
+    hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+    dfree(h->part);
+    h->part_cap = 0;
+    h->part = dalloc<double>(doubles, "hipMalloc(partial sums)");
+    h->part_cap = doubles;
+}
+
+void ensure_tickets(pv_handle *h, size_t count) {
+    if (count <= h->tickets_cap && h->tickets)
+        return;
+    hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+    dfree(h->tickets);
+    h->tickets_cap = 0;
+    h->tickets = dalloc<unsigned>(count, "hipMalloc(tile tickets)");
+    hip_check(hipMemset(h->tickets, 0, std::max<size_t>(count, 1) * sizeof(unsigned)),
+              "clear tile tickets");
+    h->tickets_cap = count;
+}
+
+void ensure_reduction(pv_handle *h, size_t doubles) {
+    if (doubles <= h->red_cap && h->red)
+        return;
+    hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+    dfree(h->red);
+    h->red_cap = 0;
+    h->red = dalloc<double>(doubles, "hipMalloc(reduction partials)");
+    h->red_cap = doubles;
+}
+
+void ensure_candidates(pv_handle *h, size_t cap) {
+    if (cap <= h->cand_cap && h->cand_first)
+        return;
+    hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+    dfree(h->cand_first), dfree(h->cand_second), dfree(h->cand_d2);
+    h->cand_cap = 0;
+    h->cand_first = dalloc<unsigned long long>(cap, "hipMalloc(dipole candidates)");
+    h->cand_second = dalloc<unsigned long long>(cap, "hipMalloc(dipole candidates)");
+    h->cand_d2 = dalloc<double>(cap, "hipMalloc(dipole candidates)");
+    h->cand_cap = cap;
+}
+
+void begin_sequence(pv_handle *h) {
+    pv::launch_reset_status(h->st, h->stream);
+    hip_check(hipGetLastError(), "launch HIP status reset");
+    h->epoch_base = h->epoch + 1;
+    h->records.clear();
+}
+
+// One velocity evaluation of positions (x, y) with circulations g, targets [tb, te).
+void evaluate(pv_handle *h, const double *x, const double *y, const double *g, size_t n,
+              size_t tb, size_t te, const pv::StageOp &op) {
+    const int prec = h->cfg.precision;
+    const size_t targets = te - tb;
+    int tpt = h->tpt_override;
+    if (tpt <= 0)
+        tpt = targets >= 32768 ? 2 : 1;
+    tpt = tpt >= 4 ? 4 : (tpt >= 2 ? 2 : 1);
+    const int tile = pv::pair_tile(prec);
+    const size_t tiles = (targets + (size_t)pv::kBlock * tpt - 1) / ((size_t)pv::kBlock * tpt);
+    const size_t max_split = std::max<size_t>(1, (n + tile - 1) / tile);
+    size_t nsplit = h->split_override > 0 ? (size_t)h->split_override : 0;
+    if (nsplit == 0) {
+        const size_t want = (size_t)4 * h->cu_count;
+        nsplit = std::max<size_t>(1, (want + tiles - 1) / tiles);
+    }
+    nsplit = std::min(nsplit, max_split);
+    size_t split_len = (n + nsplit - 1) / nsplit;
+    split_len = std::max<size_t>(tile, (split_len + tile - 1) / tile * tile);
+    nsplit = std::max<size_t>(1, (n + split_len - 1) / split_len);
+    if (nsplit > 1)
+        ensure_partials(h, nsplit * 2 * n);
+    ensure_tickets(h, tiles);
+
+    pv::PairLaunch a{};
+    a.x = x, a.y = y, a.g = g;
+    a.n = (int)n;
+    a.tbegin = (int)tb, a.tend = (int)te;
+    a.split_len = (int)split_len;
+    a.nsplit = (int)nsplit;
+    a.part = h->part;
+    a.tickets = h->tickets;
+    a.st = h->st;
+    a.epoch = ++h->epoch;
+    a.op = op;
+    h->records.push_back({x, y, (int)n, (int)tb, (int)te});
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (h->profiling) {
+        while (h->events.size() < h->events_used + 2) {
+            hipEvent_t e;
+            hip_check(hipEventCreate(&e), "create HIP event");
+            h->events.push_back(e);
+        }
+        ev0 = h->events[h->events_used++];
+        ev1 = h->events[h->events_used++];
+        hip_check(hipEventRecord(ev0, h->stream), "record HIP event");
+    }
+    pv::launch_pair(h->cfg.geometry, prec, tpt, a, h->gp, h->stream);
+    hip_check(hipGetLastError(), "launch HIP velocity kernel");
+    if (ev1)
+        hip_check(hipEventRecord(ev1, h->stream), "record HIP event");
+}
+
+pv::StageOp store_op(double *u, double *v) {
+    pv::StageOp op{};
+    op.kind = pv::kOpStore;
+    op.ku = u, op.kv = v;
+    return op;
+}
+
+const char *coincident_message(int geometry) {
+    return geometry == PV_INFINITE   ? "coincident vortices in singular point-vortex kernel"
+           : geometry == PV_PERIODIC ? "coincident periodic vortices"
+                                     : "coincident vortices in disk";
+}
+
+void read_status(pv_handle *h) {
+    hip_check(hipMemcpyAsync(h->st_host, h->st, sizeof(pv::DevStatus), hipMemcpyDeviceToHost,
+                             h->stream),
+              "read HIP status");
+    hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+}
+
+// Waits for the stream, then turns the sticky status into the reference's exception.
+// `velocity_must_be_finite` is evaluate() (true) versus evaluateRange() (false).
+void finish(pv_handle *h, bool velocity_must_be_finite) {
+    read_status(h);
+    const unsigned flags = h->st_host->flags;
+    if (!flags)
+        return;
+    if (flags & pv::kFailStateNonFinite)
+        raise(PV_ERR_INVALID_ARGUMENT, "vortex data must be finite");
+    if (flags & pv::kFailOutsideDisk)
+        raise(PV_ERR_INVALID_ARGUMENT, "vortex lies on or outside the disk");
+    if (flags & pv::kFailVelocity) {
+        const unsigned e = h->st_host->fail_epoch;
+        if (e >= h->epoch_base && e - h->epoch_base < h->records.size()) {
+            const EvalRecord r = h->records[e - h->epoch_base];
+            pv::launch_classify(h->gp, r.x, r.y, r.n, r.tbegin, r.tend, h->st, h->stream);
+            hip_check(hipGetLastError(), "launch HIP coincidence check");
+            read_status(h);
+        }
+        if (h->st_host->flags & pv::kFailCoincident)
+            raise(PV_ERR_RUNTIME, coincident_message(h->cfg.geometry));
+        if (velocity_must_be_finite)
+            raise(PV_ERR_RUNTIME, "non-finite velocity; check scales and close encounters");
+    }
+    if (flags & pv::kFailErrorEstimate)
+        raise(PV_ERR_RUNTIME, "non-finite adaptive error estimate or invalid tolerance");
+}
+
+// src/vortex.h:7-14, src/compute.cpp:31-48 on host arrays.
+void validate_host_arrays(const double *x, const double *y, const double *g, size_t n) {
+    if (n && (!x || !y || !g))
+        raise(PV_ERR_INVALID_ARGUMENT, "vortex arrays have different lengths");
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(g[i]))
+            raise(PV_ERR_INVALID_ARGUMENT, "vortex data must be finite");
+}
+
+void validate_geometry(const pv_handle *h, const double *x, const double *y, const double *g,
+                       size_t n) {
+    if (h->cfg.geometry == PV_PERIODIC) {
+        double total = 0.0, absolute = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            total += g[i];
+            absolute += std::fabs(g[i]);
+        }
+        if (std::fabs(total) > 1e-12 * std::max(1.0, absolute))
+            raise(PV_ERR_INVALID_ARGUMENT, "periodic box requires zero total circulation");
+    } else if (h->cfg.geometry == PV_DISK) {
+        const double r2 = h->cfg.disk_radius * h->cfg.disk_radius;
+        for (size_t i = 0; i < n; ++i)
+            if (x[i] * x[i] + y[i] * y[i] >= r2)
+                raise(PV_ERR_INVALID_ARGUMENT, "vortex lies on or outside the disk");
+    }
+}
+
+void require_state(const pv_handle *h) {
+    if (!h->state_valid)
+        raise(PV_ERR_LOGIC, "HIP device state has not been uploaded");
+}
+
+void require_dt(double dt) {
+    if (!std::isfinite(dt) || !(dt > 0.0))
+        raise(PV_ERR_INVALID_ARGUMENT, "timestep must be finite and positive");
+}
+
+void upload_to(pv_handle *h, double *dx, double *dy, double *dg, const double *x,
+               const double *y, const double *g, size_t n) {
+    const size_t bytes = n * sizeof(double);
+    hip_check(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, h->stream), "copy x to GPU");
+    hip_check(hipMemcpyAsync(dy, y, bytes, hipMemcpyHostToDevice, h->stream), "copy y to GPU");
+    if (dg)
+        hip_check(hipMemcpyAsync(dg, g, bytes, hipMemcpyHostToDevice, h->stream),
+                  "copy circulation to GPU");
+}
+
+// Moments {G, G x, G y, G r^2} of device arrays (src/compute.cpp:11-27).
+void device_moments(pv_handle *h, const double *x, const double *y, const double *g, size_t n,
+                    double out[4]) {
+    const int nblocks = (int)std::max<size_t>(
+        1, std::min<size_t>(1024, (n + pv::kReduceBlock - 1) / pv::kReduceBlock));
+    ensure_reduction(h, (size_t)nblocks * 8);
+    pv::launch_moments(x, y, g, (int)n, h->red, nblocks, h->stream);
+    hip_check(hipGetLastError(), "launch HIP moments kernel");
+    std::vector<double> part((size_t)nblocks * 8);
+    hip_check(hipMemcpyAsync(part.data(), h->red, part.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream),
+              "read HIP moments");
+    hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+    for (int q = 0; q < 4; ++q) {
+        double s = 0.0, c = 0.0;  // Neumaier over block partials, fixed order
+        for (int b = 0; b < nblocks; ++b)
+            for (int k = 0; k < 2; ++k) {
+                const double xq = part[(size_t)b * 8 + 2 * q + k];
+                const double t = s + xq;
+                c += std::fabs(s) >= std::fabs(xq) ? (s - t) + xq : (xq - t) + s;
+                s = t;
+            }
+        out[q] = s + c;
+    }
+    for (int q = 0; q < 4; ++q)
+        if (!std::isfinite(out[q]))
+            raise(PV_ERR_RUNTIME, "non-finite vortex moment; check input scales");
+}
+
+// Hamiltonian of device arrays (src/compute.cpp:85-87, 142-151, 174-202, 262-285).
+double device_hamiltonian(pv_handle *h, const double *x, const double *y, const double *g,
+                          size_t n) {
+    if (n == 0)
+        return 0.0;
+    const int nblocks = (int)((n + pv::kReduceBlock - 1) / pv::kReduceBlock);
+    ensure_reduction(h, (size_t)nblocks * 2);
+    begin_sequence(h);
+    pv::launch_energy(h->gp, x, y, g, (int)n, h->red, nblocks, h->st, h->stream);
+    hip_check(hipGetLastError(), "launch HIP Hamiltonian kernel");
+    std::vector<double> part((size_t)nblocks * 2);
+    hip_check(hipMemcpyAsync(part.data(), h->red, part.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, h->stream),
+              "read HIP Hamiltonian");
+    read_status(h);
+    const unsigned flags = h->st_host->flags;
+    if (flags & pv::kFailOutsideDisk)
+        raise(PV_ERR_INVALID_ARGUMENT, "vortex outside disk");
+    if (flags & pv::kFailCoincident)
+        raise(PV_ERR_RUNTIME, h->cfg.geometry == PV_INFINITE   ? "coincident vortices"
+                              : h->cfg.geometry == PV_PERIODIC ? "coincident periodic vortices"
+                                                               : "coincident vortices in disk");
+    double s = 0.0, c = 0.0;
+    for (double xq : part) {
+        const double t = s + xq;
+        c += std::fabs(s) >= std::fabs(xq) ? (s - t) + xq : (xq - t) + s;
+        s = t;
+    }
+    return s + c;
+}
+
+void set_geometry(pv_handle *h) {
+    const pv_config &c = h->cfg;
+    pv::GeoParams &p = h->gp;
+    p = pv::GeoParams{};
+    p.geometry = c.geometry;
+    if (c.geometry == PV_INFINITE) {
+        p.eps2 = c.core_radius * c.core_radius;
+    } else if (c.geometry == PV_PERIODIC) {
+        p.layers = c.image_layers;
+        p.L = c.box_length_x;
+        p.invL = 1.0 / c.box_length_x;
+        p.wave = 2.0 * pv::kPi / c.box_length_x;
+        p.scale = 0.5 / c.box_length_x;
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int n = 1; n <= pv::kMaxTerms; ++n) {
+            long double s = 0.0L;
+            for (int m = c.image_layers; m >= 1; --m)
+                s += expl(-2.0L * pi * n * m);
+            p.G4[n] = (double)(4.0L * s);
+            p.H4[n] = (double)(4.0L * s / n);
+            p.G4f[n] = (float)(4.0L * s);
+        }
+        long double cl = 0.0L;
+        for (int m = c.image_layers; m >= 1; --m)
+            cl += log1pl(expl(-4.0L * pi * m));
+        p.CL = (double)(2.0L * cl);
+    } else {
+        p.R = c.disk_radius;
+        p.invR = 1.0 / c.disk_radius;
+        p.R2 = c.disk_radius * c.disk_radius;
+    }
+}
+
+// Constructor checks of src/compute.cpp:50-56, 165-173, 258-261 and the geometry switch of
+// src/backend_common.cpp:4-13.
+void check_config(const pv_config &c) {
+    if (c.geometry == PV_INFINITE) {
+        const double r = c.core_radius, r2 = r * r;
+        if (!(r >= 0.0) || !std::isfinite(r2) || (r > 0.0 && r2 == 0.0))
+            raise(PV_ERR_INVALID_ARGUMENT,
+                  "core radius is negative or outside the supported numeric range");
+    } else if (c.geometry == PV_PERIODIC) {
+        const double lx = c.box_length_x, ly = c.box_length_y;
+        if (!(lx > 0.0) || !(ly > 0.0) || !std::isfinite(lx) || !std::isfinite(ly) ||
+            !std::isfinite(2.0 * pv::kPi / lx) || c.image_layers < 0 || c.image_layers > 64)
+            raise(PV_ERR_INVALID_ARGUMENT, "invalid periodic-box parameters");
+        if (std::fabs(lx - ly) > 1e-13 * std::max(lx, ly))
+            raise(PV_ERR_INVALID_ARGUMENT,
+                  "Weiss-McWilliams kernel requires a square periodic box");
+    } else if (c.geometry == PV_DISK) {
+        const double r = c.disk_radius, r2 = r * r;
+        if (!(r > 0.0) || !std::isfinite(r2) || r2 == 0.0)
+            raise(PV_ERR_INVALID_ARGUMENT, "invalid disk parameters");
+    } else {
+        raise(PV_ERR_INVALID_ARGUMENT, "unsupported boundary condition");
+    }
+    if (c.precision != PV_FP64 && c.precision != PV_MIXED)
+        raise(PV_ERR_INVALID_ARGUMENT, "unsupported precision");
+}
+
+pv_handle *checked(pv_handle *h) {
+    if (!h)
+        raise(PV_ERR_INVALID_ARGUMENT, "null handle");
+    return h;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *pv_last_error(void) { return g_error.c_str(); }
+
+pv_status pv_create(const pv_config *config, pv_handle **out) {
+    return guard([&] {
+        if (!config || !out)
+            raise(PV_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        check_config(*config);
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "count HIP devices");
+        if (config->device < 0 || config->device >= count)
+            raise(PV_ERR_INVALID_ARGUMENT, "invalid HIP device ordinal");
+        DeviceGuard dg(config->device);
+        auto h = std::make_unique<pv_handle>();
+        h->cfg = *config;
+        set_geometry(h.get());
+        hipDeviceProp_t prop{};
+        hip_check(hipGetDeviceProperties(&prop, config->device), "query HIP device properties");
+        h->cu_count = std::max(1, prop.multiProcessorCount);
+        hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "create HIP stream");
+        h->st = dalloc<pv::DevStatus>(1, "hipMalloc(status)");
+        hip_check(hipHostMalloc((void **)&h->st_host, sizeof(pv::DevStatus), hipHostMallocDefault),
+                  "hipHostMalloc(status)");
+        std::memset(h->st_host, 0, sizeof(pv::DevStatus));
+        begin_sequence(h.get());
+        hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+        *out = h.release();
+    });
+}
+
+void pv_destroy(pv_handle *h) {
+    if (!h)
+        return;
+    try {
+        DeviceGuard dg(h->cfg.device);
+        delete h;
+    } catch (...) {
+        delete h;
+    }
+}
+
+pv_status pv_set_precision(pv_handle *h, int precision) {
+    return guard([&] {
+        checked(h);
+        if (precision != PV_FP64 && precision != PV_MIXED)
+            raise(PV_ERR_INVALID_ARGUMENT, "unsupported precision");
+        h->cfg.precision = precision;
+    });
+}
+
+pv_status pv_set_launch(pv_handle *h, int targets_per_thread, int source_splits) {
+    return guard([&] {
+        checked(h);
+        if (targets_per_thread != 0 && targets_per_thread != 1 && targets_per_thread != 2 &&
+            targets_per_thread != 4)
+            raise(PV_ERR_INVALID_ARGUMENT, "targets per thread must be 0, 1, 2 or 4");
+        if (source_splits < 0)
+            raise(PV_ERR_INVALID_ARGUMENT, "source splits must be non-negative");
+        h->tpt_override = targets_per_thread;
+        h->split_override = source_splits;
+    });
+}
+
+static void evaluate_host(pv_handle *h, const double *x, const double *y, const double *g,
+                          size_t n, double *u, double *v, size_t begin, size_t end,
+                          bool must_be_finite) {
+    checked(h);
+    check_population(n);
+    validate_host_arrays(x, y, g, n);
+    if (begin > end || end > n)
+        raise(PV_ERR_OUT_OF_RANGE, "invalid target-vortex range");
+    validate_geometry(h, x, y, g, n);
+    if (begin == end)
+        return;
+    if (!u || !v)
+        raise(PV_ERR_INVALID_ARGUMENT, "null velocity output");
+    DeviceGuard dg(h->cfg.device);
+    ensure_scratch(h, n);
+    begin_sequence(h);
+    upload_to(h, h->sx, h->sy, h->sg, x, y, g, n);
+    evaluate(h, h->sx, h->sy, h->sg, n, begin, end, store_op(h->su, h->sv));
+    const size_t bytes = (end - begin) * sizeof(double);
+    hip_check(hipMemcpyAsync(u + begin, h->su + begin, bytes, hipMemcpyDeviceToHost, h->stream),
+              "copy u from GPU");
+    hip_check(hipMemcpyAsync(v + begin, h->sv + begin, bytes, hipMemcpyDeviceToHost, h->stream),
+              "copy v from GPU");
+    finish(h, false);
+    if (must_be_finite)
+        for (size_t i = begin; i < end; ++i)
+            if (!std::isfinite(u[i]) || !std::isfinite(v[i]))
+                raise(PV_ERR_RUNTIME, "non-finite velocity; check scales and close encounters");
+}
+
+pv_status pv_evaluate_range(pv_handle *h, const double *x, const double *y, const double *gamma,
+                            size_t n, double *u, double *v, size_t begin, size_t end) {
+    return guard([&] { evaluate_host(h, x, y, gamma, n, u, v, begin, end, false); });
+}
+
+pv_status pv_evaluate(pv_handle *h, const double *x, const double *y, const double *gamma,
+                      size_t n, double *u, double *v) {
+    return guard([&] { evaluate_host(h, x, y, gamma, n, u, v, 0, n, true); });
+}
+
+pv_status pv_hamiltonian(pv_handle *h, const double *x, const double *y, const double *gamma,
+                         size_t n, double *out) {
+    return guard([&] {
+        checked(h);
+        check_population(n);
+        if (!out)
+            raise(PV_ERR_INVALID_ARGUMENT, "null output");
+        validate_host_arrays(x, y, gamma, n);
+        DeviceGuard dg(h->cfg.device);
+        if (n == 0) {
+            *out = 0.0;
+            return;
+        }
+        ensure_scratch(h, n);
+        upload_to(h, h->sx, h->sy, h->sg, x, y, gamma, n);
+        *out = device_hamiltonian(h, h->sx, h->sy, h->sg, n);
+    });
+}
+
+pv_status pv_invariants_of(pv_handle *h, const double *x, const double *y, const double *gamma,
+                           size_t n, pv_invariants *out) {
+    return guard([&] {
+        checked(h);
+        check_population(n);
+        if (!out)
+            raise(PV_ERR_INVALID_ARGUMENT, "null output");
+        validate_host_arrays(x, y, gamma, n);
+        DeviceGuard dg(h->cfg.device);
+        double m[4] = {0, 0, 0, 0};
+        double ham = 0.0;
+        if (n) {
+            ensure_scratch(h, n);
+            upload_to(h, h->sx, h->sy, h->sg, x, y, gamma, n);
+            device_moments(h, h->sx, h->sy, h->sg, n, m);
+            ham = device_hamiltonian(h, h->sx, h->sy, h->sg, n);
+        }
+        if (!std::isfinite(ham))
+            raise(PV_ERR_RUNTIME, "non-finite Hamiltonian; check input scales");
+        *out = {m[0], m[1], m[2], m[3], ham};
+    });
+}
+
+pv_status pv_upload(pv_handle *h, const double *x, const double *y, const double *gamma,
+                    size_t n) {
+    return guard([&] {
+        checked(h);
+        check_population(n);
+        validate_host_arrays(x, y, gamma, n);
+        validate_geometry(h, x, y, gamma, n);
+        DeviceGuard dg(h->cfg.device);
+        h->fsal_valid = false;
+        if (n == 0) {
+            h->n = 0;
+            h->state_valid = true;
+            return;
+        }
+        ensure_state(h, n);
+        h->n = n;
+        h->state_valid = true;
+        upload_to(h, h->x, h->y, h->g, x, y, gamma, n);
+        hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+    });
+}
+
+pv_status pv_download(pv_handle *h, double *x, double *y, double *gamma, size_t n) {
+    return guard([&] {
+        checked(h);
+        require_state(h);
+        if (n != h->n)
+            raise(PV_ERR_RUNTIME, "host and HIP vortex populations differ");
+        if (n == 0)
+            return;
+        DeviceGuard dg(h->cfg.device);
+        const size_t bytes = n * sizeof(double);
+        hip_check(hipMemcpyAsync(x, h->x, bytes, hipMemcpyDeviceToHost, h->stream),
+                  "download state x from GPU");
+        hip_check(hipMemcpyAsync(y, h->y, bytes, hipMemcpyDeviceToHost, h->stream),
+                  "download state y from GPU");
+        if (gamma)
+            hip_check(hipMemcpyAsync(gamma, h->g, bytes, hipMemcpyDeviceToHost, h->stream),
+                      "download circulation from GPU");
+        hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(x[i]) || !std::isfinite(y[i]))
+                raise(PV_ERR_INVALID_ARGUMENT, "vortex data must be finite");
+    });
+}
+
+pv_status pv_state_size(pv_handle *h, size_t *n) {
+    return guard([&] {
+        checked(h);
+        require_state(h);
+        *n = h->n;
+    });
+}
+
+pv_status pv_evaluate_device_state(pv_handle *h, double *u, double *v, size_t n) {
+    return guard([&] {
+        checked(h);
+        require_state(h);
+        if (n != h->n)
+            raise(PV_ERR_RUNTIME, "host and HIP vortex populations differ");
+        if (h->n == 0)
+            return;
+        DeviceGuard dg(h->cfg.device);
+        begin_sequence(h);
+        evaluate(h, h->x, h->y, h->g, h->n, 0, h->n, store_op(h->u, h->v));
+        const size_t bytes = h->n * sizeof(double);
+        hip_check(hipMemcpyAsync(u, h->u, bytes, hipMemcpyDeviceToHost, h->stream),
+                  "download velocity x from GPU");
+        hip_check(hipMemcpyAsync(v, h->v, bytes, hipMemcpyDeviceToHost, h->stream),
+                  "download velocity y from GPU");
+        finish(h, true);
+    });
+}
+
+// Enqueues one RK4 step reading (x0, y0) and writing (xn, yn) (src/timestep.cpp:31-61).
+static void enqueue_rk4(pv_handle *h, double dt, const double *x0, const double *y0, double *xn,
+                        double *yn) {
+    const size_t n = h->n;
+    pv::StageOp op{};
+    op.dt = dt;
+    op.x0 = x0, op.y0 = y0;
+    for (int s = 0; s < 7; ++s)
+        op.kx[s] = h->kx[s], op.ky[s] = h->ky[s];
+    // k0 at x0 -> stage 1 position x0 + dt (1/2 k0)
+    op.kind = pv::kOpStage, op.cur = 0;
+    op.coef[0] = 0.5;
+    op.ku = h->kx[0], op.kv = h->ky[0], op.xn = h->xa, op.yn = h->ya;
+    evaluate(h, x0, y0, h->g, n, 0, n, op);
+    // k1 -> stage 2 position x0 + dt (0 k0 + 1/2 k1)
+    op.cur = 1, op.coef[0] = 0.0, op.coef[1] = 0.5;
+    op.ku = h->kx[1], op.kv = h->ky[1], op.xn = h->xb, op.yn = h->yb;
+    evaluate(h, h->xa, h->ya, h->g, n, 0, n, op);
+    // k2 -> stage 3 position x0 + dt (0 k0 + 0 k1 + 1 k2)
+    op.cur = 2, op.coef[1] = 0.0, op.coef[2] = 1.0;
+    op.ku = h->kx[2], op.kv = h->ky[2], op.xn = h->xa, op.yn = h->ya;
+    evaluate(h, h->xb, h->yb, h->g, n, 0, n, op);
+    // k3 -> x0 + dt (k0 + 2 k1 + 2 k2 + k3) / 6
+    op.kind = pv::kOpRk4Final, op.cur = 3;
+    op.ku = h->kx[3], op.kv = h->ky[3], op.xn = xn, op.yn = yn;
+    evaluate(h, h->xa, h->ya, h->g, n, 0, n, op);
+}
+
+pv_status pv_rk4_step(pv_handle *h, double dt) {
+    return guard([&] {
+        checked(h);
+        require_dt(dt);
+        require_state(h);
+        if (h->n == 0)
+            return;
+        DeviceGuard dg(h->cfg.device);
+        h->fsal_valid = false;
+        begin_sequence(h);
+        enqueue_rk4(h, dt, h->x, h->y, h->xalt, h->yalt);
+        finish(h, true);
+        std::swap(h->x, h->xalt);
+        std::swap(h->y, h->yalt);
+    });
+}
+
+pv_status pv_rk4_advance(pv_handle *h, double dt, int steps) {
+    return guard([&] {
+        checked(h);
+        require_dt(dt);
+        require_state(h);
+        if (steps < 0)
+            raise(PV_ERR_INVALID_ARGUMENT, "step count must be non-negative");
+        if (h->n == 0 || steps == 0)
+            return;
+        DeviceGuard dg(h->cfg.device);
+        h->fsal_valid = false;
+        begin_sequence(h);
+        double *bx[2] = {h->x, h->xalt}, *by[2] = {h->y, h->yalt};
+        for (int s = 0; s < steps; ++s)
+            enqueue_rk4(h, dt, bx[s & 1], by[s & 1], bx[(s + 1) & 1], by[(s + 1) & 1]);
+        try {
+            finish(h, true);
+        } catch (ApiError &e) {
+            // Keep the state of the last good step: the input of the failing one.
+            const unsigned e0 = h->st_host->fail_epoch;
+            const int failed = (int)((e0 - h->epoch_base) / 4);
+            h->x = bx[failed & 1], h->xalt = bx[(failed + 1) & 1];
+            h->y = by[failed & 1], h->yalt = by[(failed + 1) & 1];
+            e.message += " (RK4 step " + std::to_string(failed + 1) + " of " +
+                         std::to_string(steps) + ")";
+            throw;
+        }
+        h->x = bx[steps & 1], h->xalt = bx[(steps + 1) & 1];
+        h->y = by[steps & 1], h->yalt = by[(steps + 1) & 1];
+    });
+}
+
+pv_status pv_dopri5_step(pv_handle *h, double dt, double atol, double rtol, double min_dt,
+                         double max_dt, pv_step_result *out) {
+    return guard([&] {
+        checked(h);
+        require_dt(dt);
+        require_state(h);
+        if (!out)
+            raise(PV_ERR_INVALID_ARGUMENT, "null step result");
+        const size_t n = h->n;
+        unsigned rejected = 0;
+        DeviceGuard dg(h->cfg.device);
+        if (n == 0) {
+            // An empty population has zero error: src/timestep.cpp:140-149.
+            const double suggested = std::clamp(dt * 5.0, min_dt, max_dt);
+            h->fsal_valid = true;
+            *out = {dt, suggested, 0.0, 0u};
+            return;
+        }
+        // src/timestep.cpp:111-112: FSAL derivative at the current state.
+        if (!h->fsal_valid) {
+            begin_sequence(h);
+            evaluate(h, h->x, h->y, h->g, n, 0, n, store_op(h->kx[0], h->ky[0]));
+            finish(h, true);
+        }
+        for (;;) {
+            begin_sequence(h);
+            pv::StageOp op{};
+            op.dt = dt;
+            op.x0 = h->x, op.y0 = h->y;
+            op.atol = atol, op.rtol = rtol;
+            for (int s = 0; s < 7; ++s) {
+                op.kx[s] = h->kx[s], op.ky[s] = h->ky[s];
+                op.w[s] = dt * (kB5[s] - kB4[s]);
+            }
+            // Stage positions alternate between the two stage buffers; stage 6 lands in B,
+            // which becomes the state on acceptance.
+            double *px[7] = {nullptr, h->xa, h->xb, h->xa, h->xb, h->xa, h->xb};
+            double *py[7] = {nullptr, h->ya, h->yb, h->ya, h->yb, h->ya, h->yb};
+            op.kind = pv::kOpStage, op.cur = 0;
+            op.coef[0] = kDopriA[1][0];
+            op.xn = px[1], op.yn = py[1];
+            pv::launch_make_stage(op, (int)n, h->st, h->epoch, h->stream);
+            hip_check(hipGetLastError(), "launch HIP Runge--Kutta stage");
+            for (int s = 1; s < 7; ++s) {
+                op.cur = s;
+                op.ku = h->kx[s], op.kv = h->ky[s];
+                if (s < 6) {
+                    op.kind = pv::kOpStage;
+                    for (int j = 0; j <= s; ++j)
+                        op.coef[j] = kDopriA[s + 1][j];
+                    op.xn = px[s + 1], op.yn = py[s + 1];
+                } else {
+                    op.kind = pv::kOpDopriError;
+                    op.xn = nullptr, op.yn = nullptr;
+                }
+                evaluate(h, px[s], py[s], h->g, n, 0, n, op);
+            }
+            finish(h, true);
+            double error;
+            std::memcpy(&error, &h->st_host->err_bits, sizeof error);
+            // src/timestep.cpp:140-153 (controller and acceptance).
+            const double factor =
+                error == 0.0 ? 5.0 : std::clamp(0.9 * std::pow(error, -0.2), 0.2, 5.0);
+            const double suggested = std::clamp(dt * factor, min_dt, max_dt);
+            if (error <= 1.0) {
+                std::swap(h->x, h->xb);
+                std::swap(h->y, h->yb);
+                std::swap(h->kx[0], h->kx[6]);
+                std::swap(h->ky[0], h->ky[6]);
+                h->fsal_valid = true;
+                *out = {dt, suggested, error, rejected};
+                return;
+            }
+            if (dt <= min_dt || ++rejected > 32)
+                raise(PV_ERR_RUNTIME, "adaptive integrator could not satisfy tolerance");
+            dt = std::max(min_dt, std::min(suggested, dt * 0.9));
+        }
+    });
+}
+
+void pv_invalidate_fsal(pv_handle *h) {
+    if (h)
+        h->fsal_valid = false;
+}
+
+pv_status pv_state_invariants(pv_handle *h, pv_invariants *out) {
+    return guard([&] {
+        checked(h);
+        require_state(h);
+        if (!out)
+            raise(PV_ERR_INVALID_ARGUMENT, "null output");
+        DeviceGuard dg(h->cfg.device);
+        double m[4] = {0, 0, 0, 0};
+        double ham = 0.0;
+        if (h->n) {
+            device_moments(h, h->x, h->y, h->g, h->n, m);
+            ham = device_hamiltonian(h, h->x, h->y, h->g, h->n);
+        }
+        if (!std::isfinite(ham))
+            raise(PV_ERR_RUNTIME, "non-finite Hamiltonian; check input scales");
+        *out = {m[0], m[1], m[2], m[3], ham};
+    });
+}
+
+pv_status pv_dipole_candidates(pv_handle *h, double distance, uint64_t *first, uint64_t *second,
+                               double *distance_squared, size_t cap, size_t *count) {
+    return guard([&] {
+        checked(h);
+        require_state(h);
+        if (!count)
+            raise(PV_ERR_INVALID_ARGUMENT, "null count");
+        *count = 0;
+        if (h->n == 0)
+            return;
+        DeviceGuard dg(h->cfg.device);
+        const size_t dcap = std::max<size_t>(cap, 1);
+        ensure_candidates(h, dcap);
+        begin_sequence(h);
+        const double thr2 = distance * distance;
+        pv::launch_dipoles(h->gp, h->cfg.geometry == PV_PERIODIC, h->cfg.geometry == PV_DISK,
+                           thr2, h->x, h->y, h->g, (int)h->n, h->cand_first, h->cand_second,
+                           h->cand_d2, (unsigned long long)cap, h->st, h->stream);
+        hip_check(hipGetLastError(), "launch HIP dipole search");
+        read_status(h);
+        const size_t found = (size_t)h->st_host->count;
+        const size_t m = std::min(found, cap);
+        if (m) {
+            if (!first || !second || !distance_squared)
+                raise(PV_ERR_INVALID_ARGUMENT, "null candidate output");
+            hip_check(hipMemcpyAsync(first, h->cand_first, m * sizeof(uint64_t),
+                                     hipMemcpyDeviceToHost, h->stream),
+                      "read dipole candidates");
+            hip_check(hipMemcpyAsync(second, h->cand_second, m * sizeof(uint64_t),
+                                     hipMemcpyDeviceToHost, h->stream),
+                      "read dipole candidates");
+            hip_check(hipMemcpyAsync(distance_squared, h->cand_d2, m * sizeof(double),
+                                     hipMemcpyDeviceToHost, h->stream),
+                      "read dipole candidates");
+            hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+        }
+        *count = found;
+    });
+}
+
+pv_status pv_runtime_details(int device, char *buffer, size_t length) {
+    return guard([&] {
+        hipDeviceProp_t prop{};
+        hip_check(hipGetDeviceProperties(&prop, device), "query HIP device properties");
+        int runtime = 0;
+        hip_check(hipRuntimeGetVersion(&runtime), "query HIP runtime version");
+        const std::string s = "hip_device " + std::to_string(device) + "\nhip_device_name \"" +
+                              std::string(prop.name) + "\"\nhip_arch " +
+                              std::string(prop.gcnArchName) + "\nhip_compute_units " +
+                              std::to_string(prop.multiProcessorCount) + "\nhip_runtime " +
+                              std::to_string(runtime);
+        if (buffer && length) {
+            std::strncpy(buffer, s.c_str(), length - 1);
+            buffer[length - 1] = '\0';
+        }
+    });
+}
+
+pv_status pv_evaluate_resident(pv_handle *h, int repeats) {
+    return guard([&] {
+        checked(h);
+        require_state(h);
+        if (h->n == 0 || repeats <= 0)
+            return;
+        DeviceGuard dg(h->cfg.device);
+        begin_sequence(h);
+        for (int r = 0; r < repeats; ++r)
+            evaluate(h, h->x, h->y, h->g, h->n, 0, h->n, store_op(h->u, h->v));
+    });
+}
+
+pv_status pv_synchronize(pv_handle *h) {
+    return guard([&] {
+        checked(h);
+        DeviceGuard dg(h->cfg.device);
+        finish(h, true);
+    });
+}
+
+pv_status pv_profile_begin(pv_handle *h) {
+    return guard([&] {
+        checked(h);
+        h->profiling = true;
+        h->events_used = 0;
+    });
+}
+
+pv_status pv_profile_end(pv_handle *h, double *kernel_ms, int *launches) {
+    return guard([&] {
+        checked(h);
+        DeviceGuard dg(h->cfg.device);
+        hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+        double total = 0.0;
+        for (size_t k = 0; k + 1 < h->events_used; k += 2) {
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, h->events[k], h->events[k + 1]),
+                      "read HIP event time");
+            total += ms;
+        }
+        if (kernel_ms)
+            *kernel_ms = total;
+        if (launches)
+            *launches = (int)(h->events_used / 2);
+        h->profiling = false;
+        h->events_used = 0;
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,560 @@
+// pv_device.hpp -- device math and per-geometry pair-interaction policies for gfx950.
+//
+// Each policy describes one geometry in one arithmetic:
+//   Src   the per-source record staged in LDS (read by every lane of a wave at once,
+//         i.e. an LDS broadcast), with all source-only factors folded in;
+//   Tgt   the per-target record held in registers;
+//   Acc   the per-target accumulators;
+//   pair()          one source's contribution to one target (j != i guaranteed);
+//   pair_checked()  the same, for the few source tiles that contain the target itself;
+//   fold()          called after each LDS tile (mixed: FP32 tile sum -> FP64 TwoSum).
+//
+// The pair formulas restate src/compute.cpp:117-126 (infinite), 223-249 (periodic) and
+// 306-329 (disk).  The periodic image sum uses the exact Fourier form of the reference's
+// truncated row sum (see PerF64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pv {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kInvTwoPi = 0.5 / kPi;  // src/compute.cpp:97 (inverseTwoPi)
+constexpr int kMaxTerms = 16;            // capacity of the Fourier weight tables
+constexpr int kTermsF64 = 12;            // periodic Fourier terms, FP64 (tail < 2e-17)
+constexpr int kTermsF32 = 6;             // periodic Fourier terms, mixed (tail < 1e-8)
+
+// Geometry constants, filled on the host (pv_api.cpp) from pv_config.
+struct GeoParams {
+    int geometry;        // 0 infinite, 1 periodic, 2 disk
+    int layers;          // periodicImageLayers
+    double eps2;         // coreRadius^2 (infinite)
+    double L, invL;      // box length and its reciprocal (periodic)
+    double wave;         // 2 pi / L                (src/compute.cpp:213)
+    double scale;        // 0.5 / L                 (src/compute.cpp:214)
+    double R, invR, R2;  // disk radius, 1/R, R^2
+    double G4[kMaxTerms + 1];   // 4 * sum_{m=1..L} exp(-2 pi n m), n = 1..kMaxTerms
+    double H4[kMaxTerms + 1];   // G4[n] / n (Hamiltonian series)
+    double CL;                  // 2 * sum_{m=1..L} log1p(exp(-4 pi m)) (Hamiltonian)
+    float G4f[kMaxTerms + 1];
+};
+
+// ---------------------------------------------------------------- device math helpers
+
+// FP64 reciprocal: hardware v_rcp_f64 seed refined by two Newton steps (error ~1 ulp).
+__device__ __forceinline__ double rcp64(double a) {
+    double r = __builtin_amdgcn_rcp(a);
+    double e = __builtin_fma(-a, r, 1.0);
+    r = __builtin_fma(e, r, r);
+    e = __builtin_fma(-a, r, 1.0);
+    return __builtin_fma(e, r, r);
+}
+
+__device__ __forceinline__ float rcp32(float a) { return __builtin_amdgcn_rcpf(a); }
+
+// Error-free addition (Knuth's TwoSum): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+    s = a + b;
+    const double bp = s - a;
+    e = (a - (s - bp)) + (b - bp);
+}
+
+// Compensated accumulator: value = hi + lo.
+struct Comp {
+    double hi, lo;
+    __device__ __forceinline__ void zero() { hi = lo = 0.0; }
+    __device__ __forceinline__ void add(double x) {
+        double s, e;
+        two_sum(hi, x, s, e);
+        hi = s;
+        lo += e;
+    }
+    __device__ __forceinline__ double value() const { return hi + lo; }
+};
+
+// Exact IEEE remainder(d, L) for the common case (the quotient fits comfortably in a double):
+// d - n L with n = d/L rounded to nearest; fma makes the subtraction exact.  A quotient that
+// lands on the other side of a half-integer only moves the result by +-L, to which every
+// periodic formula below is invariant.
+__device__ __forceinline__ double remainder_fast(double d, double L, double invL) {
+    const double n = __builtin_rint(d * invL);
+    return __builtin_fma(-n, L, d);
+}
+
+// Taylor polynomials (coefficients 1/k!) on |h| <= pi/2 (+ a rounding margin).
+// FP64: truncation below 3e-17 relative; FP32: below 1e-9 relative.
+__device__ __forceinline__ double sinh_f64(double h, double h2) {
+    double p = 1.0 / 51090942171709440000.0;        // 1/21!
+    p = __builtin_fma(p, h2, 1.0 / 121645100408832000.0);  // 1/19!
+    p = __builtin_fma(p, h2, 1.0 / 355687428096000.0);     // 1/17!
+    p = __builtin_fma(p, h2, 1.0 / 1307674368000.0);       // 1/15!
+    p = __builtin_fma(p, h2, 1.0 / 6227020800.0);          // 1/13!
+    p = __builtin_fma(p, h2, 1.0 / 39916800.0);            // 1/11!
+    p = __builtin_fma(p, h2, 1.0 / 362880.0);              // 1/9!
+    p = __builtin_fma(p, h2, 1.0 / 5040.0);                // 1/7!
+    p = __builtin_fma(p, h2, 1.0 / 120.0);                 // 1/5!
+    p = __builtin_fma(p, h2, 1.0 / 6.0);                   // 1/3!
+    p = __builtin_fma(p, h2, 1.0);
+    return h * p;
+}
+__device__ __forceinline__ double sin_f64(double h, double h2) {
+    double p = 1.0 / 51090942171709440000.0;
+    p = __builtin_fma(p, h2, -1.0 / 121645100408832000.0);
+    p = __builtin_fma(p, h2, 1.0 / 355687428096000.0);
+    p = __builtin_fma(p, h2, -1.0 / 1307674368000.0);
+    p = __builtin_fma(p, h2, 1.0 / 6227020800.0);
+    p = __builtin_fma(p, h2, -1.0 / 39916800.0);
+    p = __builtin_fma(p, h2, 1.0 / 362880.0);
+    p = __builtin_fma(p, h2, -1.0 / 5040.0);
+    p = __builtin_fma(p, h2, 1.0 / 120.0);
+    p = __builtin_fma(p, h2, -1.0 / 6.0);
+    p = __builtin_fma(p, h2, 1.0);
+    return h * p;
+}
+__device__ __forceinline__ double cos_f64(double h2) {
+    double p = 1.0 / 1124000727777607680000.0;             // 1/22!
+    p = __builtin_fma(p, h2, -1.0 / 2432902008176640000.0);  // 1/20!
+    p = __builtin_fma(p, h2, 1.0 / 6402373705728000.0);      // 1/18!
+    p = __builtin_fma(p, h2, -1.0 / 20922789888000.0);       // 1/16!
+    p = __builtin_fma(p, h2, 1.0 / 87178291200.0);           // 1/14!
+    p = __builtin_fma(p, h2, -1.0 / 479001600.0);            // 1/12!
+    p = __builtin_fma(p, h2, 1.0 / 3628800.0);               // 1/10!
+    p = __builtin_fma(p, h2, -1.0 / 40320.0);                // 1/8!
+    p = __builtin_fma(p, h2, 1.0 / 720.0);                   // 1/6!
+    p = __builtin_fma(p, h2, -1.0 / 24.0);                   // 1/4!
+    p = __builtin_fma(p, h2, 0.5);
+    return __builtin_fma(-p, h2, 1.0);
+}
+__device__ __forceinline__ float sinh_f32(float h, float h2) {
+    float p = 1.0f / 6227020800.0f;
+    p = __builtin_fmaf(p, h2, 1.0f / 39916800.0f);
+    p = __builtin_fmaf(p, h2, 1.0f / 362880.0f);
+    p = __builtin_fmaf(p, h2, 1.0f / 5040.0f);
+    p = __builtin_fmaf(p, h2, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, h2, 1.0f / 6.0f);
+    p = __builtin_fmaf(p, h2, 1.0f);
+    return h * p;
+}
+__device__ __forceinline__ float sin_f32(float h, float h2) {
+    float p = 1.0f / 6227020800.0f;
+    p = __builtin_fmaf(p, h2, -1.0f / 39916800.0f);
+    p = __builtin_fmaf(p, h2, 1.0f / 362880.0f);
+    p = __builtin_fmaf(p, h2, -1.0f / 5040.0f);
+    p = __builtin_fmaf(p, h2, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, h2, -1.0f / 6.0f);
+    p = __builtin_fmaf(p, h2, 1.0f);
+    return h * p;
+}
+__device__ __forceinline__ float cos_f32(float h2) {
+    float p = 1.0f / 87178291200.0f;
+    p = __builtin_fmaf(p, h2, -1.0f / 479001600.0f);
+    p = __builtin_fmaf(p, h2, 1.0f / 3628800.0f);
+    p = __builtin_fmaf(p, h2, -1.0f / 40320.0f);
+    p = __builtin_fmaf(p, h2, 1.0f / 720.0f);
+    p = __builtin_fmaf(p, h2, -1.0f / 24.0f);
+    p = __builtin_fmaf(p, h2, 0.5f);
+    return __builtin_fmaf(-p, h2, 1.0f);
+}
+
+// ---------------------------------------------------------------- infinite plane
+
+// src/compute.cpp:117-126: u -= G/(2 pi) dy / (r^2 + eps^2), v += G/(2 pi) dx / (...).
+struct InfF64 {
+    struct Src {
+        double x, y, g, pad;
+    };
+    struct Tgt {
+        double x, y;
+    };
+    struct Acc {
+        double u, v;
+    };
+    static constexpr int kFoldEvery = 0;
+    __device__ static Src src(const GeoParams &, double x, double y, double g) {
+        return {x, y, kInvTwoPi * g, 0.0};
+    }
+    __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
+    __device__ static void zero(Acc &a) { a.u = a.v = 0.0; }
+    __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
+        const double dx = t.x - s.x, dy = t.y - s.y;
+        const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, p.eps2));
+        const double c = s.g * rcp64(r2);
+        a.u = __builtin_fma(-c, dy, a.u);
+        a.v = __builtin_fma(c, dx, a.v);
+    }
+    __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, Acc &a,
+                                        bool self) {
+        const double dx = t.x - s.x, dy = t.y - s.y;
+        double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, p.eps2));
+        r2 = self ? 1.0 : r2;
+        const double c = self ? 0.0 : s.g * rcp64(r2);
+        a.u = __builtin_fma(-c, dy, a.u);
+        a.v = __builtin_fma(c, dx, a.v);
+    }
+    __device__ static void fold(Acc &) {}
+    __device__ static void result(const Acc &a, double &u, double &v) {
+        u = a.u;
+        v = a.v;
+    }
+};
+
+// Mixed: FP64 difference, FP32 1/r^2 and products, FP32 tile sums folded into FP64.
+struct InfMixed {
+    struct Src {
+        double x, y;
+        float g, pad0;
+        float pad1, pad2;
+    };
+    struct Tgt {
+        double x, y;
+    };
+    struct Acc {
+        float u32, v32;
+        Comp u, v;
+    };
+    static constexpr int kFoldEvery = 1;
+    __device__ static Src src(const GeoParams &, double x, double y, double g) {
+        return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
+    }
+    __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
+    __device__ static void zero(Acc &a) {
+        a.u32 = a.v32 = 0.f;
+        a.u.zero();
+        a.v.zero();
+    }
+    __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
+        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, (float)p.eps2));
+        const float c = s.g * rcp32(r2);
+        a.u32 = __builtin_fmaf(-c, dy, a.u32);
+        a.v32 = __builtin_fmaf(c, dx, a.v32);
+    }
+    __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, Acc &a,
+                                        bool self) {
+        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+        float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, (float)p.eps2));
+        r2 = self ? 1.f : r2;
+        const float c = self ? 0.f : s.g * rcp32(r2);
+        a.u32 = __builtin_fmaf(-c, dy, a.u32);
+        a.v32 = __builtin_fmaf(c, dx, a.v32);
+    }
+    __device__ static void fold(Acc &a) {
+        a.u.add((double)a.u32);
+        a.v.add((double)a.v32);
+        a.u32 = a.v32 = 0.f;
+    }
+    __device__ static void result(const Acc &a, double &u, double &v) {
+        u = a.u.value();
+        v = a.v.value();
+    }
+};
+
+// ---------------------------------------------------------------- disk
+
+// src/compute.cpp:306-329: direct term (j != i) plus the circle-theorem image of every
+// source (j == i included).  t = x_i/R, s = x_j/R, a = 1 - t.s, b = t_y s_x - t_x s_y,
+// u -= c(-a s_y + b s_x), v += c(-a s_x - b s_y), c = -G/(2 pi R (a^2+b^2)).
+struct DiskF64 {
+    struct Src {
+        double x, y, g, sx, sy, gr;
+    };
+    struct Tgt {
+        double x, y, tx, ty;
+    };
+    struct Acc {
+        double u, v;
+    };
+    static constexpr int kFoldEvery = 0;
+    __device__ static Src src(const GeoParams &p, double x, double y, double g) {
+        return {x, y, kInvTwoPi * g, x * p.invR, y * p.invR, kInvTwoPi * g * p.invR};
+    }
+    __device__ static Tgt tgt(const GeoParams &p, double x, double y) {
+        return {x, y, x * p.invR, y * p.invR};
+    }
+    __device__ static void zero(Acc &a) { a.u = a.v = 0.0; }
+    __device__ static void image(const Tgt &t, const Src &s, Acc &a) {
+        const double a_ = 1.0 - __builtin_fma(t.tx, s.sx, t.ty * s.sy);
+        const double b_ = __builtin_fma(t.ty, s.sx, -(t.tx * s.sy));
+        const double den = __builtin_fma(a_, a_, b_ * b_);
+        const double k = s.gr * rcp64(den);
+        const double pu = __builtin_fma(a_, s.sy, -(b_ * s.sx));
+        const double pv = __builtin_fma(a_, s.sx, b_ * s.sy);
+        a.u = __builtin_fma(-k, pu, a.u);
+        a.v = __builtin_fma(k, pv, a.v);
+    }
+    __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, Acc &a) {
+        const double dx = t.x - s.x, dy = t.y - s.y;
+        const double r2 = __builtin_fma(dx, dx, dy * dy);
+        const double c = s.g * rcp64(r2);
+        a.u = __builtin_fma(-c, dy, a.u);
+        a.v = __builtin_fma(c, dx, a.v);
+        image(t, s, a);
+    }
+    __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, Acc &a,
+                                        bool self) {
+        const double dx = t.x - s.x, dy = t.y - s.y;
+        double r2 = __builtin_fma(dx, dx, dy * dy);
+        r2 = self ? 1.0 : r2;
+        const double c = self ? 0.0 : s.g * rcp64(r2);
+        a.u = __builtin_fma(-c, dy, a.u);
+        a.v = __builtin_fma(c, dx, a.v);
+        image(t, s, a);
+    }
+    __device__ static void fold(Acc &) {}
+    __device__ static void result(const Acc &a, double &u, double &v) {
+        u = a.u;
+        v = a.v;
+    }
+};
+
+// Mixed disk.  With d = x_i - x_j formed in FP64 and w = 1 - |t|^2 formed in FP64 per target,
+// a = w + (t/R).d and b = (t/R) x d are the same a, b as above (a = 1 - t.(t - d/R)) but
+// without the cancellation of 1 - t.s near the wall, so they can be evaluated in FP32.
+struct DiskMixed {
+    struct Src {
+        double x, y;
+        float g, sx, sy, gr;
+    };
+    struct Tgt {
+        double x, y;
+        float trx, try_, w, pad;
+    };
+    struct Acc {
+        float u32, v32;
+        Comp u, v;
+    };
+    static constexpr int kFoldEvery = 1;
+    __device__ static Src src(const GeoParams &p, double x, double y, double g) {
+        return {x, y, (float)(kInvTwoPi * g), (float)(x * p.invR), (float)(y * p.invR),
+                (float)(kInvTwoPi * g * p.invR)};
+    }
+    __device__ static Tgt tgt(const GeoParams &p, double x, double y) {
+        const double tx = x * p.invR, ty = y * p.invR;
+        const double w = 1.0 - __builtin_fma(tx, tx, ty * ty);
+        return {x, y, (float)(tx * p.invR), (float)(ty * p.invR), (float)w, 0.f};
+    }
+    __device__ static void zero(Acc &a) {
+        a.u32 = a.v32 = 0.f;
+        a.u.zero();
+        a.v.zero();
+    }
+    __device__ static void body(const Tgt &t, const Src &s, Acc &a, float cdir) {
+        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+        const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+        const float c = cdir * rcp32(r2);
+        a.u32 = __builtin_fmaf(-c, dy, a.u32);
+        a.v32 = __builtin_fmaf(c, dx, a.v32);
+        const float a_ = __builtin_fmaf(t.trx, dx, __builtin_fmaf(t.try_, dy, t.w));
+        const float b_ = __builtin_fmaf(t.trx, dy, -(t.try_ * dx));
+        const float den = __builtin_fmaf(a_, a_, b_ * b_);
+        const float k = s.gr * rcp32(den);
+        const float pu = __builtin_fmaf(a_, s.sy, -(b_ * s.sx));
+        const float pv = __builtin_fmaf(a_, s.sx, b_ * s.sy);
+        a.u32 = __builtin_fmaf(-k, pu, a.u32);
+        a.v32 = __builtin_fmaf(k, pv, a.v32);
+    }
+    __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, Acc &a) {
+        body(t, s, a, s.g);
+    }
+    __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, Acc &a,
+                                        bool self) {
+        // The self pair has no direct term; its own image has d = 0, so a = w and b = 0.
+        if (self) {
+            const float a_ = t.w;
+            const float k = s.gr * rcp32(a_ * a_);
+            a.u32 = __builtin_fmaf(-k, a_ * s.sy, a.u32);
+            a.v32 = __builtin_fmaf(k, a_ * s.sx, a.v32);
+        } else {
+            body(t, s, a, s.g);
+        }
+    }
+    __device__ static void fold(Acc &a) {
+        a.u.add((double)a.u32);
+        a.v.add((double)a.v32);
+        a.u32 = a.v32 = 0.f;
+    }
+    __device__ static void result(const Acc &a, double &u, double &v) {
+        u = a.u.value();
+        v = a.v.value();
+    }
+};
+
+// ---------------------------------------------------------------- periodic box
+
+// src/compute.cpp:223-249.  With X = k rem(dx, L), Y = k rem(dy, L) in [-pi, pi]:
+//   u -= scale G [ sin Y / (cosh X - cos Y) + sum_{m != 0, |m| <= L} sin Y / (cosh(X - 2 pi m) - cos Y) ]
+// (and v with X, Y exchanged).  The m = 0 row keeps the reference's cancellation-free
+// denominator 2(sinh^2(X/2) + sin^2(Y/2)).  For |X| < 2 pi the image rows have the exact
+// expansion  sum_{m != 0} ... = 4 sum_{n>=1} G_n sin(nY) cosh(nX),  G_n = sum_{m=1..L} e^{-2 pi n m},
+// evaluated with the Chebyshev recurrences for sin(nY) and cosh(nX).  Terms fall as
+// e^{-pi n}; 12 terms reach 2e-17 (FP64), 6 terms 1e-8 (mixed).  The reference drops image
+// terms with |X - 2 pi m| > 40 (below 5e-18 relative); the series keeps them.
+struct PerF64 {
+    struct Src {
+        double x, y, gs, pad;
+    };
+    struct Tgt {
+        double x, y;
+    };
+    struct Acc {
+        double u, v;
+    };
+    static constexpr int kFoldEvery = 0;
+    __device__ static Src src(const GeoParams &p, double x, double y, double g) {
+        return {x, y, p.scale * g, 0.0};
+    }
+    __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
+    __device__ static void zero(Acc &a) { a.u = a.v = 0.0; }
+    __device__ static void terms(const GeoParams &p, const Tgt &t, const Src &s, double &tu,
+                                 double &tv, double &d0u, double &d0v, double &sin_x,
+                                 double &sin_y) {
+        const double X = p.wave * remainder_fast(t.x - s.x, p.L, p.invL);
+        const double Y = p.wave * remainder_fast(t.y - s.y, p.L, p.invL);
+        const double hx = 0.5 * X, hy = 0.5 * Y;
+        const double hx2 = hx * hx, hy2 = hy * hy;
+        const double shx = sinh_f64(hx, hx2), shy = sinh_f64(hy, hy2);
+        const double snx = sin_f64(hx, hx2), sny = sin_f64(hy, hy2);
+        const double csx = cos_f64(hx2), csy = cos_f64(hy2);
+        const double snx2 = snx * snx, sny2 = sny * sny;
+        const double shx2 = shx * shx, shy2 = shy * shy;
+        sin_x = 2.0 * snx * csx;
+        sin_y = 2.0 * sny * csy;
+        const double cos2x = __builtin_fma(-4.0, snx2, 2.0);   // 2 cos X
+        const double cos2y = __builtin_fma(-4.0, sny2, 2.0);
+        const double ch2x = __builtin_fma(4.0, shx2, 2.0);     // 2 cosh X
+        const double ch2y = __builtin_fma(4.0, shy2, 2.0);
+        d0u = 2.0 * (shx2 + sny2);
+        d0v = 2.0 * (shy2 + snx2);
+        // sin(nY) cosh(nX) and sin(nX) cosh(nY), n = 1..kTermsF64.
+        double sy0 = 0.0, sy1 = sin_y, sx0 = 0.0, sx1 = sin_x;
+        double cx0 = 1.0, cx1 = 0.5 * ch2x, cy0 = 1.0, cy1 = 0.5 * ch2y;
+        double au = 0.0, av = 0.0;
+#pragma unroll
+        for (int n = 1; n <= kTermsF64; ++n) {
+            au = __builtin_fma(p.G4[n] * sy1, cx1, au);
+            av = __builtin_fma(p.G4[n] * sx1, cy1, av);
+            if (n < kTermsF64) {
+                const double sy2 = __builtin_fma(cos2y, sy1, -sy0);
+                const double sx2 = __builtin_fma(cos2x, sx1, -sx0);
+                const double cx2 = __builtin_fma(ch2x, cx1, -cx0);
+                const double cy2 = __builtin_fma(ch2y, cy1, -cy0);
+                sy0 = sy1, sy1 = sy2, sx0 = sx1, sx1 = sx2;
+                cx0 = cx1, cx1 = cx2, cy0 = cy1, cy1 = cy2;
+            }
+        }
+        tu = au;
+        tv = av;
+    }
+    __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
+        double tu, tv, d0u, d0v, sx, sy;
+        terms(p, t, s, tu, tv, d0u, d0v, sx, sy);
+        const double fu = __builtin_fma(sy, rcp64(d0u), tu);
+        const double fv = __builtin_fma(sx, rcp64(d0v), tv);
+        a.u = __builtin_fma(-s.gs, fu, a.u);
+        a.v = __builtin_fma(s.gs, fv, a.v);
+    }
+    __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, Acc &a,
+                                        bool self) {
+        double tu, tv, d0u, d0v, sx, sy;
+        terms(p, t, s, tu, tv, d0u, d0v, sx, sy);
+        d0u = self ? 1.0 : d0u;
+        d0v = self ? 1.0 : d0v;
+        const double fu = __builtin_fma(sy, rcp64(d0u), tu);
+        const double fv = __builtin_fma(sx, rcp64(d0v), tv);
+        a.u = __builtin_fma(-s.gs, fu, a.u);
+        a.v = __builtin_fma(s.gs, fv, a.v);
+    }
+    __device__ static void fold(Acc &) {}
+    __device__ static void result(const Acc &a, double &u, double &v) {
+        u = a.u;
+        v = a.v;
+    }
+};
+
+// Mixed periodic: the remainder and scaling in FP64, everything after in FP32.
+struct PerMixed {
+    struct Src {
+        double x, y;
+        float gs, pad0, pad1, pad2;
+    };
+    struct Tgt {
+        double x, y;
+    };
+    struct Acc {
+        float u32, v32;
+        Comp u, v;
+    };
+    static constexpr int kFoldEvery = 1;
+    __device__ static Src src(const GeoParams &p, double x, double y, double g) {
+        return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
+    }
+    __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
+    __device__ static void zero(Acc &a) {
+        a.u32 = a.v32 = 0.f;
+        a.u.zero();
+        a.v.zero();
+    }
+    __device__ static void terms(const GeoParams &p, const Tgt &t, const Src &s, float &fu,
+                                 float &fv, bool self) {
+        const float X = (float)(p.wave * remainder_fast(t.x - s.x, p.L, p.invL));
+        const float Y = (float)(p.wave * remainder_fast(t.y - s.y, p.L, p.invL));
+        const float hx = 0.5f * X, hy = 0.5f * Y;
+        const float hx2 = hx * hx, hy2 = hy * hy;
+        const float shx = sinh_f32(hx, hx2), shy = sinh_f32(hy, hy2);
+        const float snx = sin_f32(hx, hx2), sny = sin_f32(hy, hy2);
+        const float csx = cos_f32(hx2), csy = cos_f32(hy2);
+        const float snx2 = snx * snx, sny2 = sny * sny;
+        const float shx2 = shx * shx, shy2 = shy * shy;
+        const float sin_x = 2.f * snx * csx, sin_y = 2.f * sny * csy;
+        const float cos2x = __builtin_fmaf(-4.f, snx2, 2.f);
+        const float cos2y = __builtin_fmaf(-4.f, sny2, 2.f);
+        const float ch2x = __builtin_fmaf(4.f, shx2, 2.f);
+        const float ch2y = __builtin_fmaf(4.f, shy2, 2.f);
+        float d0u = 2.f * (shx2 + sny2);
+        float d0v = 2.f * (shy2 + snx2);
+        float sy0 = 0.f, sy1 = sin_y, sx0 = 0.f, sx1 = sin_x;
+        float cx0 = 1.f, cx1 = 0.5f * ch2x, cy0 = 1.f, cy1 = 0.5f * ch2y;
+        float au = 0.f, av = 0.f;
+#pragma unroll
+        for (int n = 1; n <= kTermsF32; ++n) {
+            au = __builtin_fmaf(p.G4f[n] * sy1, cx1, au);
+            av = __builtin_fmaf(p.G4f[n] * sx1, cy1, av);
+            if (n < kTermsF32) {
+                const float sy2 = __builtin_fmaf(cos2y, sy1, -sy0);
+                const float sx2 = __builtin_fmaf(cos2x, sx1, -sx0);
+                const float cx2 = __builtin_fmaf(ch2x, cx1, -cx0);
+                const float cy2 = __builtin_fmaf(ch2y, cy1, -cy0);
+                sy0 = sy1, sy1 = sy2, sx0 = sx1, sx1 = sx2;
+                cx0 = cx1, cx1 = cx2, cy0 = cy1, cy1 = cy2;
+            }
+        }
+        d0u = self ? 1.f : d0u;
+        d0v = self ? 1.f : d0v;
+        fu = __builtin_fmaf(sin_y, rcp32(d0u), au);
+        fv = __builtin_fmaf(sin_x, rcp32(d0v), av);
+    }
+    __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
+        float fu, fv;
+        terms(p, t, s, fu, fv, false);
+        a.u32 = __builtin_fmaf(-s.gs, fu, a.u32);
+        a.v32 = __builtin_fmaf(s.gs, fv, a.v32);
+    }
+    __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, Acc &a,
+                                        bool self) {
+        float fu, fv;
+        terms(p, t, s, fu, fv, self);
+        a.u32 = __builtin_fmaf(-s.gs, fu, a.u32);
+        a.v32 = __builtin_fmaf(s.gs, fv, a.v32);
+    }
+    __device__ static void fold(Acc &a) {
+        a.u.add((double)a.u32);
+        a.v.add((double)a.v32);
+        a.u32 = a.v32 = 0.f;
+    }
+    __device__ static void result(const Acc &a, double &u, double &v) {
+        u = a.u.value();
+        v = a.v.value();
+    }
+};
+
+} // namespace pv

@@ -1,0 +1,578 @@
+// pv_kernels.hip -- gfx950 kernels of the velocity backend.
+//
+// Pair kernel layout (one launch per velocity evaluation):
+//   grid.x  = target tiles of kBlock*TPT targets; each lane holds TPT targets in registers,
+//             strided by kBlock so that target loads are coalesced;
+//   grid.y  = source splits of split_len sources; each block walks its split in LDS tiles of
+//             TILE source records (SoA x, y, G in HBM -> derived records in LDS), which every
+//             lane of a wave then reads at the same address (an LDS broadcast);
+//   splits  > 1 write FP64 partial sums; the last block of a target tile (atomic ticket) adds
+//             them in split order, so every target's sum has a fixed order (deterministic).
+// The stage arithmetic of RK4 / DOPRI5 is fused into the epilogue of the evaluation that
+// produces the stage (StageOp), so an RK4 step is four launches and a DOPRI5 attempt seven.
+#include "pv_kernels.hpp"
+
+#include <climits>
+
+namespace pv {
+
+__device__ __forceinline__ bool finite64(double v) { return __builtin_isfinite(v); }
+
+__device__ __forceinline__ void record_failure(DevStatus *st, unsigned epoch, unsigned kind) {
+    atomicOr(&st->flags, kind);
+    atomicMin(&st->fail_epoch, epoch);
+}
+
+template <typename T> __device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Max over the block; result valid in every thread.
+__device__ double block_max(double v) {
+    __shared__ double red[kBlock / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0)
+        red[wave] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+        m = fmax(m, red[w]);
+    return m;
+}
+
+// Per-target epilogue; returns this target's DOPRI5 error contribution (0 otherwise).
+__device__ double epilogue(const PairLaunch &a, const GeoParams &gp, int i, double u,
+                           double v) {
+    const StageOp &op = a.op;
+    if (!finite64(u) || !finite64(v))
+        record_failure(a.st, a.epoch, kFailVelocity);
+    if (op.ku) {
+        op.ku[i] = u;
+        op.kv[i] = v;
+    }
+    if (op.kind == kOpStage) {
+        // src/timestep.cpp:19-30 (makeStage), coefficients in stage order.
+        double dx = 0.0, dy = 0.0;
+        for (int j = 0; j < op.cur; ++j) {
+            dx += op.coef[j] * op.kx[j][i];
+            dy += op.coef[j] * op.ky[j][i];
+        }
+        dx += op.coef[op.cur] * u;
+        dy += op.coef[op.cur] * v;
+        op.xn[i] = op.x0[i] + op.dt * dx;
+        op.yn[i] = op.y0[i] + op.dt * dy;
+    } else if (op.kind == kOpRk4Final) {
+        // src/timestep.cpp:48-58: combination, then the finiteness check of the new state.
+        const double xn = op.x0[i] + op.dt * (op.kx[0][i] + 2.0 * op.kx[1][i] +
+                                              2.0 * op.kx[2][i] + u) / 6.0;
+        const double yn = op.y0[i] + op.dt * (op.ky[0][i] + 2.0 * op.ky[1][i] +
+                                              2.0 * op.ky[2][i] + v) / 6.0;
+        op.xn[i] = xn;
+        op.yn[i] = yn;
+        if (!finite64(xn) || !finite64(yn))
+            record_failure(a.st, a.epoch, kFailStateNonFinite);
+    } else if (op.kind == kOpDopriError) {
+        // src/timestep.cpp:83-98: max-norm of the embedded error, scaled per component.
+        double ex = 0.0, ey = 0.0;
+        for (int s = 0; s < 6; ++s) {
+            ex += op.w[s] * op.kx[s][i];
+            ey += op.w[s] * op.ky[s][i];
+        }
+        ex += op.w[6] * u;
+        ey += op.w[6] * v;
+        const double x5 = a.x[i], y5 = a.y[i];
+        const double scx = op.atol + op.rtol * fmax(fabs(op.x0[i]), fabs(x5));
+        const double scy = op.atol + op.rtol * fmax(fabs(op.y0[i]), fabs(y5));
+        if (!finite64(ex) || !finite64(ey) || !finite64(scx) || !finite64(scy) || !(scx > 0.0) ||
+            !(scy > 0.0)) {
+            record_failure(a.st, a.epoch, kFailErrorEstimate);
+            return __builtin_inf();
+        }
+        return fmax(fabs(ex) / scx, fabs(ey) / scy);
+    }
+    return 0.0;
+}
+
+template <class P, int TPT, int TILE>
+__global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp) {
+    if (a.st->fail_epoch < a.epoch)
+        return;
+    __shared__ typename P::Src sh[TILE];
+    __shared__ int last_block;
+    const int tid = threadIdx.x;
+    constexpr int TT = kBlock * TPT;
+    const int tile = blockIdx.x;
+    const int split = blockIdx.y;
+    const int t0 = a.tbegin + tile * TT;
+
+    typename P::Tgt tg[TPT];
+    typename P::Acc acc[TPT];
+    int ti[TPT];
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = t0 + t * kBlock + tid;
+        const bool valid = i < a.tend;
+        const int ii = valid ? i : a.tbegin;
+        const double x = a.x[ii], y = a.y[ii];
+        if (valid && split == 0) {
+            // src/vortex.h:11-13 and src/compute.cpp:41-48 for every evaluated target.
+            if (!finite64(x) || !finite64(y))
+                record_failure(a.st, a.epoch, kFailStateNonFinite);
+            else if (gp.geometry == 2 && x * x + y * y >= gp.R2)
+                record_failure(a.st, a.epoch, kFailOutsideDisk);
+        }
+        tg[t] = P::tgt(gp, x, y);
+        P::zero(acc[t]);
+        ti[t] = valid ? i : -1;
+    }
+
+    const int s0 = split * a.split_len;
+    const int s1 = min(a.n, s0 + a.split_len);
+    for (int base = s0; base < s1; base += TILE) {
+        const int cnt = min(TILE, s1 - base);
+        __syncthreads();
+        for (int k = tid; k < cnt; k += kBlock) {
+            const int j = base + k;
+            sh[k] = P::src(gp, a.x[j], a.y[j], a.g[j]);
+        }
+        __syncthreads();
+        const bool diag = base < t0 + TT && t0 < base + cnt;
+        if (!diag) {
+            int k = 0;
+            for (; k + 2 <= cnt; k += 2) {
+                const typename P::Src s_a = sh[k];
+                const typename P::Src s_b = sh[k + 1];
+#pragma unroll
+                for (int t = 0; t < TPT; ++t)
+                    P::pair(gp, tg[t], s_a, acc[t]);
+#pragma unroll
+                for (int t = 0; t < TPT; ++t)
+                    P::pair(gp, tg[t], s_b, acc[t]);
+            }
+            if (k < cnt) {
+                const typename P::Src s_a = sh[k];
+#pragma unroll
+                for (int t = 0; t < TPT; ++t)
+                    P::pair(gp, tg[t], s_a, acc[t]);
+            }
+        } else {
+            for (int k = 0; k < cnt; ++k) {
+                const typename P::Src s_a = sh[k];
+                const int j = base + k;
+#pragma unroll
+                for (int t = 0; t < TPT; ++t)
+                    P::pair_checked(gp, tg[t], s_a, acc[t], ti[t] == j);
+            }
+        }
+        if (P::kFoldEvery) {
+#pragma unroll
+            for (int t = 0; t < TPT; ++t)
+                P::fold(acc[t]);
+        }
+    }
+
+    double u[TPT], v[TPT];
+#pragma unroll
+    for (int t = 0; t < TPT; ++t)
+        P::result(acc[t], u[t], v[t]);
+
+    if (a.nsplit > 1) {
+        const size_t n = (size_t)a.n;
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) {
+            if (ti[t] >= 0) {
+                a.part[(size_t)(2 * split) * n + ti[t]] = u[t];
+                a.part[(size_t)(2 * split + 1) * n + ti[t]] = v[t];
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            const unsigned prev = atomicAdd(&a.tickets[tile], 1u);
+            last_block = prev == (unsigned)(a.nsplit - 1);
+        }
+        __syncthreads();
+        if (!last_block)
+            return;
+        __threadfence();
+#pragma unroll
+        for (int t = 0; t < TPT; ++t) {
+            if (ti[t] >= 0) {
+                Comp su, sv;
+                su.zero();
+                sv.zero();
+                for (int s = 0; s < a.nsplit; ++s) {
+                    su.add(__builtin_nontemporal_load(&a.part[(size_t)(2 * s) * n + ti[t]]));
+                    sv.add(__builtin_nontemporal_load(&a.part[(size_t)(2 * s + 1) * n + ti[t]]));
+                }
+                u[t] = su.value();
+                v[t] = sv.value();
+            }
+        }
+        if (tid == 0)
+            a.tickets[tile] = 0u;
+    }
+
+    double err = 0.0;
+#pragma unroll
+    for (int t = 0; t < TPT; ++t)
+        if (ti[t] >= 0)
+            err = fmax(err, epilogue(a, gp, ti[t], u[t], v[t]));
+    if (a.op.kind == kOpDopriError) {
+        const double m = block_max(err);
+        if (tid == 0)
+            atomicMax(&a.st->err_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// ------------------------------------------------------------------ small kernels
+
+__global__ void reset_status_kernel(DevStatus *st) {
+    if (threadIdx.x == 0) {
+        st->flags = 0u;
+        st->fail_epoch = UINT_MAX;
+        st->err_bits = 0ull;
+        st->count = 0ull;
+    }
+}
+
+// Stage positions from stored stage velocities only (DOPRI5 stage 1).
+__global__ void make_stage_kernel(StageOp op, int n, const DevStatus *st, unsigned epoch) {
+    if (st->fail_epoch < epoch)
+        return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double dx = 0.0, dy = 0.0;
+    for (int j = 0; j <= op.cur; ++j) {
+        dx += op.coef[j] * op.kx[j][i];
+        dy += op.coef[j] * op.ky[j][i];
+    }
+    op.xn[i] = op.x0[i] + op.dt * dx;
+    op.yn[i] = op.y0[i] + op.dt * dy;
+}
+
+// Exact zero-denominator test of the reference's loops (src/compute.cpp:119-123, 234-246,
+// 307-313), with its non-fused arithmetic.  Runs only after a non-finite result, to tell a
+// coincident pair apart from an overflow.
+__global__ void classify_kernel(GeoParams gp, const double *x, const double *y, int n,
+                                int tbegin, int tend, DevStatus *st) {
+    const int i = tbegin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tend)
+        return;
+    bool hit = false;
+    for (int j = 0; j < n && !hit; ++j) {
+        if (j == i)
+            continue;
+        if (gp.geometry == 1) {
+            const double sdx = __dmul_rn(gp.wave, remainder(x[i] - x[j], gp.L));
+            const double sdy = __dmul_rn(gp.wave, remainder(y[i] - y[j], gp.L));
+            const double shx = sinh(0.5 * sdx), shy = sinh(0.5 * sdy);
+            const double snx = sin(0.5 * sdx), sny = sin(0.5 * sdy);
+            const double du = 2.0 * __dadd_rn(__dmul_rn(shx, shx), __dmul_rn(sny, sny));
+            const double dv = 2.0 * __dadd_rn(__dmul_rn(shy, shy), __dmul_rn(snx, snx));
+            hit = du == 0.0 || dv == 0.0;
+        } else {
+            const double dx = __dsub_rn(x[i], x[j]), dy = __dsub_rn(y[i], y[j]);
+            double d = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+            if (gp.geometry == 0)
+                d = __dadd_rn(d, gp.eps2);
+            hit = d == 0.0;
+        }
+    }
+    if (hit)
+        atomicOr(&st->flags, (unsigned)kFailCoincident);
+}
+
+// Block partial sums of {G, G x, G y, G r^2} (src/compute.cpp:11-27), compensated; each
+// block writes 8 doubles (hi, lo pairs) in a fixed order.
+__global__ void moments_kernel(const double *x, const double *y, const double *g, int n,
+                               double *partials) {
+    __shared__ double red[kReduceBlock][8];
+    Comp c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        c[q].zero();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double gam = g[i];
+        c[0].add(gam);
+        c[1].add(gam * x[i]);
+        c[2].add(gam * y[i]);
+        c[3].add(gam * (x[i] * x[i] + y[i] * y[i]));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        red[threadIdx.x][2 * q] = c[q].hi;
+        red[threadIdx.x][2 * q + 1] = c[q].lo;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int q = threadIdx.x;
+        Comp s;
+        s.zero();
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+            s.add(red[t][2 * q]);
+            s.add(red[t][2 * q + 1]);
+        }
+        partials[blockIdx.x * 8 + 2 * q] = s.hi;
+        partials[blockIdx.x * 8 + 2 * q + 1] = s.lo;
+    }
+}
+
+// Pair energy e_ij of the Hamiltonians (symmetric in i, j):
+//   infinite  log(r^2 + eps^2)                                      src/compute.cpp:142-151
+//   disk      log(|d|^2 R^2 / ((R^2 - x_i.x_j)^2 + (x_i x x_j)^2))   src/compute.cpp:271-282
+//   periodic  -X^2/2pi + sum_{|m|<=L} [log D_m - log cosh 2 pi m]   src/compute.cpp:182-199
+// with the periodic rows m != 0 summed exactly as -4 sum_n G_n cos(nY) cosh(nX) / n - C_L.
+__device__ double pair_energy(const GeoParams &gp, double xi, double yi, double xj, double yj,
+                              bool &zero) {
+    if (gp.geometry == 0) {
+        const double dx = xi - xj, dy = yi - yj;
+        const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, gp.eps2));
+        zero = r2 == 0.0;
+        return log(r2);
+    }
+    if (gp.geometry == 2) {
+        const double dx = xi - xj, dy = yi - yj;
+        const double num = __builtin_fma(dx, dx, dy * dy) * gp.R2;
+        const double dot = __builtin_fma(xi, xj, yi * yj);
+        const double cross = __builtin_fma(yi, xj, -(xi * yj));
+        const double den = __builtin_fma(gp.R2 - dot, gp.R2 - dot, cross * cross);
+        zero = num == 0.0;
+        return log(num) - log(den);
+    }
+    const double X = gp.wave * remainder_fast(xi - xj, gp.L, gp.invL);
+    const double Y = gp.wave * remainder_fast(yi - yj, gp.L, gp.invL);
+    const double hx = 0.5 * X, hy = 0.5 * Y;
+    const double hx2 = hx * hx, hy2 = hy * hy;
+    const double shx = sinh_f64(hx, hx2);
+    const double sny = sin_f64(hy, hy2);
+    const double shx2 = shx * shx, sny2 = sny * sny;
+    const double d0 = 2.0 * (shx2 + sny2);
+    zero = d0 == 0.0;
+    const double cos2y = __builtin_fma(-4.0, sny2, 2.0);
+    const double ch2x = __builtin_fma(4.0, shx2, 2.0);
+    double c0 = 1.0, c1 = 0.5 * cos2y, h0 = 1.0, h1 = 0.5 * ch2x, acc = 0.0;
+#pragma unroll
+    for (int k = 1; k <= kTermsF64; ++k) {
+        acc = __builtin_fma(gp.H4[k] * c1, h1, acc);
+        const double c2 = __builtin_fma(cos2y, c1, -c0);
+        const double h2 = __builtin_fma(ch2x, h1, -h0);
+        c0 = c1, c1 = c2, h0 = h1, h1 = h2;
+    }
+    return -X * X / (2.0 * kPi) + log(d0) - acc - gp.CL;
+}
+
+__global__ __launch_bounds__(kReduceBlock) void energy_kernel(GeoParams gp, const double *x,
+                                                              const double *y, const double *g,
+                                                              int n, double *partials,
+                                                              DevStatus *st) {
+    __shared__ double sx[kReduceBlock], sy[kReduceBlock], sg[kReduceBlock];
+    __shared__ double red[kReduceBlock][2];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0;
+    Comp e;
+    e.zero();
+    bool coincident = false;
+    for (int base = 0; base < n; base += kReduceBlock) {
+        const int cnt = min(kReduceBlock, n - base);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            sx[threadIdx.x] = x[base + threadIdx.x];
+            sy[threadIdx.x] = y[base + threadIdx.x];
+            sg[threadIdx.x] = g[base + threadIdx.x];
+        }
+        __syncthreads();
+        if (valid) {
+            for (int k = 0; k < cnt; ++k) {
+                if (base + k == i)
+                    continue;
+                bool z = false;
+                const double eij = pair_energy(gp, xi, yi, sx[k], sy[k], z);
+                coincident |= z;
+                e.add(sg[k] * eij);
+            }
+        }
+    }
+    double h = 0.0, hl = 0.0;
+    if (valid) {
+        const double gi = g[i];
+        Comp hc;
+        hc.zero();
+        hc.add(-gi / (8.0 * kPi) * e.hi);
+        hc.add(-gi / (8.0 * kPi) * e.lo);
+        if (gp.geometry == 2) {
+            const double ri2 = xi * xi + yi * yi;
+            if (ri2 >= gp.R2)
+                atomicOr(&st->flags, (unsigned)kFailOutsideDisk);
+            hc.add(gi * gi * log((gp.R2 - ri2) / gp.R) / (4.0 * kPi));
+        }
+        if (coincident)
+            atomicOr(&st->flags, (unsigned)kFailCoincident);
+        h = hc.hi;
+        hl = hc.lo;
+    }
+    red[threadIdx.x][0] = h;
+    red[threadIdx.x][1] = hl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Comp s;
+        s.zero();
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+            s.add(red[t][0]);
+            s.add(red[t][1]);
+        }
+        partials[2 * blockIdx.x] = s.hi;
+        partials[2 * blockIdx.x + 1] = s.lo;
+    }
+}
+
+// Close opposite-sign pairs (src/dipole.cpp:90-118), with the reference's non-fused
+// arithmetic so that candidate sets and their d^2 match it bit for bit.
+__global__ __launch_bounds__(kReduceBlock) void dipole_kernel(
+    GeoParams gp, int periodic, int disk, double thr2, const double *x, const double *y,
+    const double *g, int n, unsigned long long *first, unsigned long long *second, double *d2,
+    unsigned long long cap, DevStatus *st) {
+    __shared__ double sx[kReduceBlock], sy[kReduceBlock], sg[kReduceBlock];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, gi = valid ? g[i] : 0.0;
+    const bool active = valid && gi != 0.0;
+    const int first_j = blockIdx.x * blockDim.x;   // sources below the block never pair j > i
+    for (int base = first_j; base < n; base += kReduceBlock) {
+        const int cnt = min(kReduceBlock, n - base);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            sx[threadIdx.x] = x[base + threadIdx.x];
+            sy[threadIdx.x] = y[base + threadIdx.x];
+            sg[threadIdx.x] = g[base + threadIdx.x];
+        }
+        __syncthreads();
+        if (!active)
+            continue;
+        for (int k = 0; k < cnt; ++k) {
+            const int j = base + k;
+            if (j <= i)
+                continue;
+            const double gj = sg[k];
+            if (gj == 0.0 || signbit(gi) == signbit(gj))
+                continue;
+            double dx = __dsub_rn(xi, sx[k]), dy = __dsub_rn(yi, sy[k]);
+            if (periodic) {
+                dx = remainder_fast(dx, gp.L, gp.invL);
+                dy = remainder_fast(dy, gp.L, gp.invL);
+            }
+            const double dd = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+            if (dd < thr2) {
+                const unsigned long long slot = atomicAdd(&st->count, 1ull);
+                if (slot < cap) {
+                    first[slot] = (unsigned long long)i;
+                    second[slot] = (unsigned long long)j;
+                    d2[slot] = dd;
+                }
+            }
+        }
+    }
+    if (active && disk) {
+        const double ri2 = __dadd_rn(__dmul_rn(xi, xi), __dmul_rn(yi, yi));
+        if (ri2 > 0.0) {
+            const double idist = (gp.R2 - ri2) / sqrt(ri2);
+            const double dd = __dmul_rn(idist, idist);
+            if (dd < thr2) {
+                const unsigned long long slot = atomicAdd(&st->count, 1ull);
+                if (slot < cap) {
+                    first[slot] = (unsigned long long)i;
+                    second[slot] = (unsigned long long)n;
+                    d2[slot] = dd;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <class P, int TPT, int TILE>
+static void launch_pair_t(const PairLaunch &a, const GeoParams &gp, hipStream_t s) {
+    const int targets = a.tend - a.tbegin;
+    const int tiles = (targets + kBlock * TPT - 1) / (kBlock * TPT);
+    dim3 grid(tiles, a.nsplit);
+    hipLaunchKernelGGL((pair_kernel<P, TPT, TILE>), grid, dim3(kBlock), 0, s, a, gp);
+}
+
+template <class P, int TILE>
+static void launch_pair_tpt(int tpt, const PairLaunch &a, const GeoParams &gp, hipStream_t s) {
+    if (tpt >= 4)
+        launch_pair_t<P, 4, TILE>(a, gp, s);
+    else if (tpt == 2)
+        launch_pair_t<P, 2, TILE>(a, gp, s);
+    else
+        launch_pair_t<P, 1, TILE>(a, gp, s);
+}
+
+int pair_tile(int precision) { return precision ? kTileMixed : kTileF64; }
+
+void launch_pair(int geometry, int precision, int tpt, const PairLaunch &a, const GeoParams &gp,
+                 hipStream_t s) {
+    if (precision == 0) {
+        if (geometry == 0)
+            launch_pair_tpt<InfF64, kTileF64>(tpt, a, gp, s);
+        else if (geometry == 1)
+            launch_pair_tpt<PerF64, kTileF64>(tpt, a, gp, s);
+        else
+            launch_pair_tpt<DiskF64, kTileF64>(tpt, a, gp, s);
+    } else {
+        if (geometry == 0)
+            launch_pair_tpt<InfMixed, kTileMixed>(tpt, a, gp, s);
+        else if (geometry == 1)
+            launch_pair_tpt<PerMixed, kTileMixed>(tpt, a, gp, s);
+        else
+            launch_pair_tpt<DiskMixed, kTileMixed>(tpt, a, gp, s);
+    }
+}
+
+void launch_make_stage(const StageOp &op, int n, const DevStatus *st, unsigned epoch,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(make_stage_kernel, dim3((n + 255) / 256), dim3(256), 0, s, op, n, st,
+                       epoch);
+}
+
+void launch_reset_status(DevStatus *st, hipStream_t s) {
+    hipLaunchKernelGGL(reset_status_kernel, dim3(1), dim3(64), 0, s, st);
+}
+
+void launch_classify(const GeoParams &gp, const double *x, const double *y, int n, int tbegin,
+                     int tend, DevStatus *st, hipStream_t s) {
+    const int cnt = tend - tbegin;
+    hipLaunchKernelGGL(classify_kernel, dim3((cnt + 255) / 256), dim3(256), 0, s, gp, x, y, n,
+                       tbegin, tend, st);
+}
+
+void launch_moments(const double *x, const double *y, const double *g, int n, double *partials,
+                    int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(moments_kernel, dim3(nblocks), dim3(kReduceBlock), 0, s, x, y, g, n,
+                       partials);
+}
+
+void launch_energy(const GeoParams &gp, const double *x, const double *y, const double *g, int n,
+                   double *partials, int nblocks, DevStatus *st, hipStream_t s) {
+    hipLaunchKernelGGL(energy_kernel, dim3(nblocks), dim3(kReduceBlock), 0, s, gp, x, y, g, n,
+                       partials, st);
+}
+
+void launch_dipoles(const GeoParams &gp, bool periodic, bool disk, double thr2, const double *x,
+                    const double *y, const double *g, int n, unsigned long long *first,
+                    unsigned long long *second, double *d2, unsigned long long cap,
+                    DevStatus *st, hipStream_t s) {
+    hipLaunchKernelGGL(dipole_kernel, dim3((n + kReduceBlock - 1) / kReduceBlock),
+                       dim3(kReduceBlock), 0, s, gp, (int)periodic, (int)disk, thr2, x, y, g, n,
+                       first, second, d2, cap, st);
+}
+
+} // namespace pv

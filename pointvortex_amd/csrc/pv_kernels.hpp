@@ -1,0 +1,79 @@
+// pv_kernels.hpp -- launch-side declarations shared by the kernels (pv_kernels.hip) and the
+// C-ABI (pv_api.cpp).
+#pragma once
+#include "pv_device.hpp"
+
+namespace pv {
+
+// Sticky device status word, one per handle.  Kernels of a launch whose epoch is later than
+// fail_epoch return at once, so the buffers stay as they were when the failure happened.
+struct DevStatus {
+    unsigned flags;                // FailureKind bits
+    unsigned fail_epoch;           // epoch of the first failing launch (UINT_MAX: none)
+    unsigned long long err_bits;   // DOPRI5 max-norm error (bit pattern of a double >= 0)
+    unsigned long long count;      // dipole candidates found
+    unsigned long long pad[4];
+};
+
+enum FailureKind : unsigned {
+    kFailStateNonFinite = 1u,    // "vortex data must be finite"
+    kFailOutsideDisk = 2u,       // "vortex lies on or outside the disk"
+    kFailVelocity = 4u,          // non-finite velocity (coincident pairs end up here too)
+    kFailErrorEstimate = 8u,     // "non-finite adaptive error estimate or invalid tolerance"
+    kFailCoincident = 16u,       // set by the classification pass
+};
+
+enum OpKind : int { kOpStore = 0, kOpStage = 1, kOpRk4Final = 2, kOpDopriError = 3 };
+
+// Fused per-target epilogue of a velocity evaluation (src/timestep.cpp:19-30, 48-57, 62-119).
+struct StageOp {
+    int kind;
+    int cur;                   // stage index of this evaluation
+    double dt;
+    double coef[7];            // kOpStage: next position = x0 + dt * sum_{j<=cur} coef[j] k_j
+    double w[7];               // kOpDopriError: dt (b5 - b4)
+    double atol, rtol;
+    double *ku, *kv;           // where this evaluation's velocity is stored (may be null)
+    const double *kx[7], *ky[7];  // earlier stage velocities
+    const double *x0, *y0;     // positions at the start of the step
+    double *xn, *yn;           // next positions
+};
+
+struct PairLaunch {
+    const double *x, *y, *g;   // positions evaluated and circulations
+    int n;                     // population; sources are [0, n)
+    int tbegin, tend;          // targets
+    int split_len;             // sources per split (multiple of the LDS tile)
+    int nsplit;
+    double *part;              // [nsplit][2][n] partial sums when nsplit > 1
+    unsigned *tickets;         // one per target tile, zero between launches
+    DevStatus *st;
+    unsigned epoch;
+    StageOp op;
+};
+
+constexpr int kBlock = 256;
+constexpr int kTileF64 = 256;
+constexpr int kTileMixed = 128;
+
+// Launchers (pv_kernels.hip).  `tpt` is 1, 2 or 4.
+void launch_pair(int geometry, int precision, int tpt, const PairLaunch &a, const GeoParams &gp,
+                 hipStream_t s);
+int pair_tile(int precision);
+void launch_make_stage(const StageOp &op, int n, const DevStatus *st, unsigned epoch,
+                       hipStream_t s);
+void launch_reset_status(DevStatus *st, hipStream_t s);
+void launch_classify(const GeoParams &gp, const double *x, const double *y, int n, int tbegin,
+                     int tend, DevStatus *st, hipStream_t s);
+void launch_moments(const double *x, const double *y, const double *g, int n, double *partials,
+                    int nblocks, hipStream_t s);
+void launch_energy(const GeoParams &gp, const double *x, const double *y, const double *g, int n,
+                   double *partials, int nblocks, DevStatus *st, hipStream_t s);
+void launch_dipoles(const GeoParams &gp, bool periodic, bool disk, double thr2, const double *x,
+                    const double *y, const double *g, int n, unsigned long long *first,
+                    unsigned long long *second, double *d2, unsigned long long cap,
+                    DevStatus *st, hipStream_t s);
+
+constexpr int kReduceBlock = 256;
+
+} // namespace pv

@@ -1,0 +1,380 @@
+"""Parity of the HIP backend (through the C-ABI) against the CPU FP64 oracle.
+
+Tolerances:
+* FP64 velocities: normwise relative error <= 1e-12 (the reference's own FP64 error against
+  an exact sum is ~1e-14 normwise); per-vortex relative error (floor 1e-3 rms) <= 1e-10.
+* mixed velocities: normwise <= 1e-6 against the FP64 oracle.
+* Hamiltonians: relative 1e-11 (sums with cancellation), moments 1e-13.
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import spec
+from pointvortex_amd import (BackendRuntimeError, HipKernel, InvalidArgument, LogicError,
+                             OutOfRange, SimParams, VortexSystem)
+from pointvortex_amd import workloads as wl
+from tests.conftest import normwise, per_vortex
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2.0 * np.pi
+
+
+def kernel_for(geometry, precision="fp64", **kw):
+    p = SimParams(boundaryCondition=geometry, precision=precision)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return HipKernel(p)
+
+
+def spec_for(geometry, **kw):
+    return spec(geometry, core_radius=kw.get("coreRadius", 0.0),
+                box_length=kw.get("boxLengthX", 2.0), image_layers=kw.get("periodicImageLayers", 8),
+                radius=kw.get("diskRadius", 1.0))
+
+
+def population(geometry, n, seed):
+    if geometry == "infinite":
+        return wl.infinite(n, seed)
+    if geometry == "periodic":
+        return wl.periodic(n, seed, box_length=2.0)
+    return wl.disk(n, seed)
+
+
+# ------------------------------------------------------------------ known answers
+
+def test_single_and_pair_known_answers():
+    """src/tests.cpp:26-41: single vortex at rest; co-rotating pair v = -+1/(4 pi)."""
+    k = kernel_for("infinite")
+    vel = k.evaluate(VortexSystem([2.0], [-3.0], [4.0]))
+    assert vel.x[0] == 0.0 and vel.y[0] == 0.0
+    vel = k.evaluate(VortexSystem([-1.0, 1.0], [0.0, 0.0], [1.0, 1.0]))
+    speed = 1.0 / (4.0 * np.pi)
+    assert abs(vel.y[0] + speed) <= 1e-15 and abs(vel.y[1] - speed) <= 1e-15
+
+
+def test_periodic_translation_invariance():
+    """src/tests.cpp:126-140 (L=2, 10 layers)."""
+    k = kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0, periodicImageLayers=10)
+    s = VortexSystem([-0.4, 0.4], [0.1, -0.1], [1.0, -1.0])
+    a = k.evaluate(s)
+    s2 = VortexSystem([-0.4 + 2.0, 0.4], [0.1, -0.1 - 2.0], [1.0, -1.0])
+    b = k.evaluate(s2)
+    assert np.max(np.abs(a.x - b.x)) <= 2e-14 and np.max(np.abs(a.y - b.y)) <= 2e-14
+
+
+def test_disk_single_vortex_image():
+    """src/tests.cpp:141-153: r = 0.5, R = 2: u = 0, v = r / (2 pi (R^2 - r^2))."""
+    k = kernel_for("disk", diskRadius=2.0)
+    vel = k.evaluate(VortexSystem([0.5], [0.0], [1.0]))
+    assert abs(vel.x[0]) <= 1e-15
+    assert abs(vel.y[0] - 0.5 / (2.0 * np.pi * (4.0 - 0.25))) <= 1e-15
+
+
+def test_close_encounters():
+    """src/audit_tests.cpp:27-47."""
+    inv2pi = 0.5 / np.pi
+    k = kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0)
+    vel = k.evaluate(VortexSystem([0.0, 1e-10], [0.0, 0.0], [1.0, -1.0]))
+    assert abs(vel.y[0] / (inv2pi / 1e-10) - 1.0) <= 1e-12
+    with pytest.raises(BackendRuntimeError, match="coincident periodic vortices"):
+        k.evaluate(VortexSystem([0.0, 0.0], [0.0, 0.0], [1.0, -1.0]))
+    d = kernel_for("disk", diskRadius=1.0)
+    vel = d.evaluate(VortexSystem([0.2, 1e-155], [0.0, 0.0], [1.0, 1.0]))
+    assert abs(vel.x[0]) <= 1e-15
+    assert abs(vel.y[0] - inv2pi * (5 + 0.2 / 0.96)) <= 1e-14
+    assert abs(vel.y[1] - inv2pi * (-5 + 0.2)) <= 1e-14
+    with pytest.raises(InvalidArgument):
+        kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0, periodicImageLayers=65)
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_hamiltonian_gradient_and_conservation(geometry):
+    """src/audit_tests.cpp:51-85: velocity = (1/G)(dH/dy, -dH/dx) by central differences, and
+    H conserved over 100 RK4 steps at dt = 1e-4."""
+    kw = {"coreRadius": 0.05} if geometry == "infinite" else {}
+    if geometry == "periodic":
+        kw = {"boxLengthX": 2.0, "boxLengthY": 2.0}
+    k = kernel_for(geometry, **kw)
+    s = VortexSystem([-.31, .23, .11, -.14], [-.12, .14, -.29, .32], [1, -1, 2, -2])
+    vel = k.evaluate(s)
+    h = 1e-6
+    for i in range(4):
+        p, m = s.copy(), s.copy()
+        p.x[i] += h
+        m.x[i] -= h
+        dx = (k.hamiltonian(p) - k.hamiltonian(m)) / (2 * h)
+        p, m = s.copy(), s.copy()
+        p.y[i] += h
+        m.y[i] -= h
+        dy = (k.hamiltonian(p) - k.hamiltonian(m)) / (2 * h)
+        assert abs(vel.x[i] - dy / s.circulation[i]) <= 2e-8
+        assert abs(vel.y[i] + dx / s.circulation[i]) <= 2e-8
+    h0 = k.hamiltonian(s)
+    k.uploadDeviceState(s)
+    for _ in range(100):
+        k.deviceRk4Step(1e-4)
+    e = s.copy()
+    k.downloadDeviceState(e)
+    assert abs(k.hamiltonian(e) - h0) <= 1e-10
+
+
+def test_invalid_inputs():
+    """src/audit_tests.cpp:99-118 and the reference's validation messages."""
+    k = kernel_for("infinite")
+    with pytest.raises(InvalidArgument, match="vortex data must be finite"):
+        k.evaluate(VortexSystem([np.nan], [0.0], [1.0]))
+    with pytest.raises(InvalidArgument, match="timestep must be finite and positive"):
+        k.uploadDeviceState(VortexSystem([0.0], [0.0], [1.0]))
+        k.deviceRk4Step(0.0)
+    with pytest.raises(OutOfRange, match="invalid target-vortex range"):
+        k.evaluateRange([0.0, 1.0], [0.0, 0.0], [1.0, 1.0], 1, 3)
+    with pytest.raises(BackendRuntimeError,
+                       match="coincident vortices in singular point-vortex kernel"):
+        k.evaluate(VortexSystem([0.5, 0.5, 0.1], [0.2, 0.2, 0.3], [1.0, -1.0, 1.0]))
+    p = kernel_for("periodic")
+    with pytest.raises(InvalidArgument, match="periodic box requires zero total circulation"):
+        p.evaluate(VortexSystem([0.0, 0.5], [0.0, 0.1], [1.0, 1.0]))
+    d = kernel_for("disk")
+    with pytest.raises(InvalidArgument, match="vortex lies on or outside the disk"):
+        d.evaluate(VortexSystem([1.0, 0.0], [0.0, 0.1], [1.0, 1.0]))
+    with pytest.raises(BackendRuntimeError, match="coincident vortices in disk"):
+        d.evaluate(VortexSystem([0.1, 0.1], [0.2, 0.2], [1.0, 1.0]))
+    fresh = kernel_for("infinite")
+    with pytest.raises(LogicError):
+        fresh.deviceRk4Step(1e-3)
+    with pytest.raises(InvalidArgument):
+        kernel_for("infinite", coreRadius=-1.0)
+    with pytest.raises(InvalidArgument, match="square periodic box"):
+        kernel_for("periodic", boxLengthX=2.0, boxLengthY=3.0)
+
+
+# ------------------------------------------------------------------ seeded parity
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+@pytest.mark.parametrize("n", [2, 4, 258, 1000, 4096])
+def test_fp64_velocity_parity(oracle, geometry, n):
+    pop = population(geometry, n, seed=11 + n)
+    k = kernel_for(geometry, boxLengthX=2.0, boxLengthY=2.0)
+    vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+    ur, vr = oracle.evaluate(spec_for(geometry, boxLengthX=2.0), pop.x, pop.y, pop.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= 1e-12
+    assert per_vortex(vel.x, vel.y, ur, vr) <= 1e-10
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+@pytest.mark.parametrize("n", [258, 4096])
+def test_mixed_velocity_parity(oracle, geometry, n):
+    pop = population(geometry, n, seed=5 + n)
+    k = kernel_for(geometry, "mixed", boxLengthX=2.0, boxLengthY=2.0)
+    vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+    ur, vr = oracle.evaluate(spec_for(geometry, boxLengthX=2.0), pop.x, pop.y, pop.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= 1e-6
+
+
+@pytest.mark.parametrize("layers", [0, 1, 3, 6, 8, 20])
+def test_periodic_layers_parity(oracle, layers):
+    pop = wl.periodic(300, 77, box_length=TWO_PI)
+    k = kernel_for("periodic", boxLengthX=TWO_PI, boxLengthY=TWO_PI, periodicImageLayers=layers)
+    vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+    ur, vr = oracle.evaluate(spec("periodic", box_length=TWO_PI, image_layers=layers), pop.x,
+                             pop.y, pop.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= 1e-12
+
+
+def test_periodic_positions_outside_box(oracle):
+    """The reference never wraps positions; remainder() reduces every difference."""
+    pop = wl.periodic(200, 9, box_length=2.0)
+    x = pop.x + 2.0 * np.random.default_rng(1).integers(-3, 4, 200)
+    k = kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0)
+    vel = k.evaluate(VortexSystem(x, pop.y, pop.circulation))
+    ur, vr = oracle.evaluate(spec("periodic", box_length=2.0), x, pop.y, pop.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= 1e-12
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_evaluate_range_subsets(oracle, geometry):
+    pop = population(geometry, 700, seed=3)
+    k = kernel_for(geometry, boxLengthX=2.0, boxLengthY=2.0)
+    s = spec_for(geometry, boxLengthX=2.0)
+    for b, e in [(0, 1), (13, 300), (650, 700), (0, 700), (5, 5)]:
+        vel = k.evaluateRange(pop.x, pop.y, pop.circulation, b, e)
+        ur, vr = oracle.evaluate_range(s, pop.x, pop.y, pop.circulation, b, e)
+        if e > b:
+            assert normwise(vel.x[b:e], vel.y[b:e], ur[b:e], vr[b:e]) <= 1e-12
+        assert np.all(vel.x[:b] == 0.0) and np.all(vel.x[e:] == 0.0)
+
+
+@pytest.mark.parametrize("tpt,splits", [(1, 1), (1, 3), (2, 0), (4, 0), (4, 7), (2, 64)])
+def test_launch_shapes_agree(oracle, tpt, splits):
+    pop = wl.disk(3000, 21)
+    k = kernel_for("disk")
+    k.set_launch(tpt, splits)
+    vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+    ur, vr = oracle.evaluate(spec("disk"), pop.x, pop.y, pop.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= 1e-12
+
+
+def test_deterministic_repeats():
+    pop = wl.periodic(2000, 4, box_length=2.0)
+    k = kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0)
+    s = VortexSystem(pop.x, pop.y, pop.circulation)
+    a, b = k.evaluate(s), k.evaluate(s)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
+# ------------------------------------------------------------------ stepping
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_rk4_steps_parity(oracle, geometry):
+    pop = population(geometry, 500, seed=8)
+    k = kernel_for(geometry, boxLengthX=2.0, boxLengthY=2.0)
+    k.uploadDeviceState(VortexSystem(pop.x, pop.y, pop.circulation))
+    dt = 1e-4 if geometry == "disk" else 1e-3
+    for _ in range(10):
+        k.deviceRk4Step(dt)
+    out = VortexSystem(pop.x.copy(), pop.y.copy(), pop.circulation)
+    k.downloadDeviceState(out)
+    it = oracle.integrator(500)
+    x, y = pop.x.copy(), pop.y.copy()
+    s = spec_for(geometry, boxLengthX=2.0)
+    for _ in range(10):
+        it.rk4_step(s, x, y, pop.circulation, dt)
+    scale = np.max(np.abs(np.concatenate([x, y])))
+    assert np.max(np.abs(out.x - x)) <= 1e-12 * scale
+    assert np.max(np.abs(out.y - y)) <= 1e-12 * scale
+
+
+def test_rk4_advance_matches_single_steps():
+    pop = wl.infinite(1024, 0)
+    a = kernel_for("infinite")
+    b = kernel_for("infinite")
+    s = VortexSystem(pop.x, pop.y, pop.circulation)
+    a.uploadDeviceState(s)
+    b.uploadDeviceState(s)
+    for _ in range(7):
+        a.deviceRk4Step(1e-3)
+    b.deviceRk4Advance(1e-3, 7)
+    ra, rb = s.copy(), s.copy()
+    a.downloadDeviceState(ra)
+    b.downloadDeviceState(rb)
+    assert np.array_equal(ra.x, rb.x) and np.array_equal(ra.y, rb.y)
+
+
+def test_rk4_pair_rotation():
+    """src/tests.cpp:42-51: co-rotating pair to t=1 at dt=1e-3 (2e-12)."""
+    k = kernel_for("infinite")
+    s = VortexSystem([-1.0, 1.0], [0.0, 0.0], [1.0, 1.0])
+    k.uploadDeviceState(s)
+    k.deviceRk4Advance(1e-3, 1000)
+    k.downloadDeviceState(s)
+    ang = 1.0 / (4.0 * np.pi)
+    assert abs(s.x[1] - np.cos(ang)) <= 2e-12 and abs(s.y[1] - np.sin(ang)) <= 2e-12
+
+
+def test_dopri_pair_rotation():
+    """src/tests.cpp:52-70: DOPRI5 to t=1 (2e-10)."""
+    k = kernel_for("infinite")
+    s = VortexSystem([-1.0, 1.0], [0.0, 0.0], [1.0, 1.0])
+    k.uploadDeviceState(s)
+    t, dt = 0.0, 0.01
+    while t < 1.0:
+        r = k.deviceDopri5Step(min(dt, 1.0 - t), 1e-12, 1e-10, 1e-12, 0.2)
+        t += r.acceptedTimeStep
+        dt = r.suggestedTimeStep
+    k.downloadDeviceState(s)
+    ang = 1.0 / (4.0 * np.pi)
+    assert abs(s.x[1] - np.cos(ang)) <= 2e-10 and abs(s.y[1] - np.sin(ang)) <= 2e-10
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_dopri5_parity(oracle, geometry):
+    pop = population(geometry, 300, seed=12)
+    k = kernel_for(geometry, boxLengthX=2.0, boxLengthY=2.0)
+    k.uploadDeviceState(VortexSystem(pop.x, pop.y, pop.circulation))
+    it = oracle.integrator(300)
+    s = spec_for(geometry, boxLengthX=2.0)
+    x, y = pop.x.copy(), pop.y.copy()
+    dt_g = dt_o = 1e-3
+    for _ in range(6):
+        rg = k.deviceDopri5Step(dt_g, 1e-12, 1e-9, 1e-12, 0.1)
+        ro = it.dopri5_step(s, x, y, pop.circulation, dt_o, 1e-12, 1e-9, 1e-12, 0.1)
+        assert rg.rejectedSteps == ro[3]
+        assert abs(rg.acceptedTimeStep - ro[0]) <= 1e-9 * ro[0]
+        assert abs(rg.normalizedError - ro[2]) <= 1e-6 * max(ro[2], 1e-300)
+        dt_g, dt_o = rg.suggestedTimeStep, ro[1]
+    out = VortexSystem(x.copy(), y.copy(), pop.circulation)
+    k.downloadDeviceState(out)
+    scale = np.max(np.abs(np.concatenate([x, y])))
+    assert np.max(np.abs(out.x - x)) <= 1e-11 * scale
+
+
+def test_dopri_fsal_counts_and_invalidation(oracle):
+    """FSAL reuse (src/tests.cpp:297-314): after invalidation the first stage is recomputed;
+    results match the oracle's integrator driven the same way."""
+    pop = wl.infinite(64, 3)
+    k = kernel_for("infinite")
+    k.uploadDeviceState(VortexSystem(pop.x, pop.y, pop.circulation))
+    it = oracle.integrator(64)
+    x, y = pop.x.copy(), pop.y.copy()
+    s = spec("infinite")
+    for step in range(3):
+        if step == 2:
+            k.invalidateDeviceDerivative()
+            it.invalidate()
+        rg = k.deviceDopri5Step(1e-3, 1e-10, 1e-8, 1e-12, 0.1)
+        ro = it.dopri5_step(s, x, y, pop.circulation, 1e-3, 1e-10, 1e-8, 1e-12, 0.1)
+        assert abs(rg.normalizedError - ro[2]) <= 1e-6 * ro[2]
+
+
+def test_dopri_nonfinite_and_empty():
+    k = kernel_for("infinite")
+    k.uploadDeviceState(VortexSystem([], [], []))
+    r = k.deviceDopri5Step(0.01, 1e-10, 1e-8, 1e-12, 0.1)
+    assert r.acceptedTimeStep == 0.01 and r.suggestedTimeStep == 0.05
+    k.uploadDeviceState(VortexSystem([0.0, 0.0], [0.0, 0.0], [1.0, 1.0]))
+    with pytest.raises(BackendRuntimeError):
+        k.deviceDopri5Step(0.01, 1e-10, 1e-8, 1e-12, 0.1)
+
+
+# ------------------------------------------------------------------ invariants, dipoles
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+@pytest.mark.parametrize("n", [3, 500, 2048])
+def test_invariants_parity(oracle, geometry, n):
+    if geometry == "periodic" and n % 2:
+        n += 1
+    pop = population(geometry, n, seed=40 + n)
+    kw = {"coreRadius": 0.01} if geometry == "infinite" else {}
+    k = kernel_for(geometry, boxLengthX=2.0, boxLengthY=2.0, **kw)
+    inv = k.invariantsOf(VortexSystem(pop.x, pop.y, pop.circulation))
+    ref = oracle.invariants(spec_for(geometry, boxLengthX=2.0, **kw), pop.x, pop.y,
+                            pop.circulation)
+    got = np.array([inv.circulation, inv.linearImpulseX, inv.linearImpulseY, inv.angularImpulse,
+                    inv.hamiltonian])
+    scale = np.sum(np.abs(pop.circulation)) * max(1.0, np.max(np.abs(pop.x)) ** 2)
+    assert np.all(np.abs(got[:4] - ref[:4]) <= 1e-13 * scale)
+    assert abs(got[4] - ref[4]) <= 1e-11 * max(1.0, abs(ref[4])) * max(1.0, np.sqrt(n))
+    k.uploadDeviceState(VortexSystem(pop.x, pop.y, pop.circulation))
+    dev = k.deviceInvariants()
+    assert dev.hamiltonian == inv.hamiltonian
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_dipole_candidates_parity(oracle, geometry):
+    pop = population(geometry, 3000, seed=90)
+    if geometry == "disk":
+        pop.circulation[:] = np.where(np.arange(3000) % 2 == 0, 1.0, -1.0)
+    dist = 0.02
+    k = kernel_for(geometry, boxLengthX=2.0, boxLengthY=2.0)
+    k.uploadDeviceState(VortexSystem(pop.x, pop.y, pop.circulation))
+    f, s, d2 = k.dipoleCandidates(dist)
+    fo, so, d2o = oracle.dipole_candidates(geometry, 2.0, 1.0, dist, pop.x, pop.y,
+                                           pop.circulation)
+    assert len(fo) > 0
+    got = sorted(zip(d2.tolist(), f.tolist(), s.tolist()))
+    want = sorted(zip(d2o.tolist(), fo.tolist(), so.tolist()))
+    assert got == want

@@ -1,0 +1,287 @@
+"""Pins the CPU oracle (oracle/pv_oracle.c) before it is trusted as the checker.
+
+* golden vectors written by the reference's own CPU code (tests/golden/*.npz);
+* the reference's closed-form known-answer tests (src/tests.cpp, src/audit_tests.cpp);
+* high-precision brute force (mpmath) of the textbook formulas on tiny inputs: Biot-Savart,
+  the circle-theorem image and, for the periodic box, rows of point vortices summed directly;
+* the reference build itself (oracle/_ref), when present, on fresh random inputs;
+* planted mistakes in a copy of the oracle's formulas must fail these checks.
+"""
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import spec
+from pointvortex_amd import workloads as wl
+from tests.conftest import normwise
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+GEOMETRIES = ["infinite", "periodic", "disk"]
+
+
+def golden(geometry):
+    return np.load(GOLDEN / f"reference_{geometry}.npz")
+
+
+@pytest.mark.parametrize("geometry", GEOMETRIES)
+@pytest.mark.parametrize("n", [2, 4, 258, 1024])
+def test_oracle_velocities_match_golden(oracle, geometry, n):
+    d = golden(geometry)
+    t = f"n{n}"
+    k = spec(geometry, box_length=2.0)
+    u, v = oracle.evaluate(k, d[f"{t}_x"], d[f"{t}_y"], d[f"{t}_g"])
+    assert normwise(u, v, d[f"{t}_u"], d[f"{t}_v"]) <= 1e-15
+
+
+@pytest.mark.parametrize("geometry", GEOMETRIES)
+@pytest.mark.parametrize("n", [2, 4, 258, 1024])
+def test_oracle_invariants_match_golden(oracle, geometry, n):
+    d = golden(geometry)
+    t = f"n{n}"
+    k = spec(geometry, core_radius=float(d[f"{t}_core"][0]), box_length=2.0)
+    inv = oracle.invariants(k, d[f"{t}_x"], d[f"{t}_y"], d[f"{t}_g"])
+    ref = d[f"{t}_invariants_core"]
+    assert np.allclose(inv, ref, rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("geometry", GEOMETRIES)
+@pytest.mark.parametrize("n", [4, 258])
+def test_oracle_integrators_match_golden(oracle, geometry, n):
+    d = golden(geometry)
+    t = f"n{n}"
+    k = spec(geometry, box_length=2.0)
+    g = d[f"{t}_g"]
+    x, y = d[f"{t}_x"].copy(), d[f"{t}_y"].copy()
+    it = oracle.integrator(n)
+    dt = float(d[f"{t}_rk4_dt"][0])
+    for _ in range(10):
+        it.rk4_step(k, x, y, g, dt)
+    assert np.max(np.abs(x - d[f"{t}_rk4_x"])) <= 1e-15
+    assert np.max(np.abs(y - d[f"{t}_rk4_y"])) <= 1e-15
+    # DOPRI5 from t=0 to 0.01, dt0 = 1e-3, landing on the end time.
+    x, y = d[f"{t}_x"].copy(), d[f"{t}_y"].copy()
+    it = oracle.integrator(n)
+    rec = d[f"{t}_dopri_record"]
+    time, dt, rows = 0.0, 1e-3, []
+    while time < 0.01:
+        r = it.dopri5_step(k, x, y, g, min(dt, 0.01 - time), 1e-12, 1e-9, 1e-12, 0.1)
+        rows.append(r)
+        time += r[0]
+        dt = r[1]
+    assert np.allclose(np.array(rows), rec, rtol=1e-12, atol=0)
+    assert np.max(np.abs(x - d[f"{t}_dopri_x"])) <= 1e-15
+
+
+@pytest.mark.parametrize("geometry", GEOMETRIES)
+def test_oracle_dipole_candidates_match_golden_removal(oracle, geometry):
+    """The pairs the reference removed are exactly the oracle's closest-first disjoint
+    selection of its candidates (src/dipole.cpp:120-141)."""
+    d = golden(geometry)
+    x, y, g = d["dipole_in_x"], d["dipole_in_y"], d["dipole_in_g"]
+    f, s, d2 = oracle.dipole_candidates(geometry, 2.0, 1.0, 0.03, x, y, g)
+    order = sorted(range(len(f)), key=lambda i: (d2[i], f[i], s[i]))
+    n = len(x)
+    sel = np.zeros(n, dtype=bool)
+    events = 0
+    for i in order:
+        a, b = int(f[i]), int(s[i])
+        if sel[a] or (b < n and sel[b]):
+            continue
+        sel[a] = True
+        if b < n:
+            sel[b] = True
+        events += 1
+    assert events == int(d["dipole_events"][0])
+    assert np.array_equal(x[~sel], d["dipole_out_x"])
+    assert np.array_equal(g[~sel], d["dipole_out_g"])
+
+
+# ------------------------------------------------------------------ closed forms
+
+
+def test_known_answers(oracle):
+    """src/tests.cpp:26-41, 126-153 and src/audit_tests.cpp:27-47 on the oracle."""
+    u, v = oracle.evaluate(spec("infinite"), [2.0], [-3.0], [4.0])
+    assert u[0] == 0.0 and v[0] == 0.0
+    u, v = oracle.evaluate(spec("infinite"), [-1.0, 1.0], [0.0, 0.0], [1.0, 1.0])
+    assert abs(v[0] + 1 / (4 * np.pi)) <= 1e-15 and abs(v[1] - 1 / (4 * np.pi)) <= 1e-15
+    u, v = oracle.evaluate(spec("disk", radius=2.0), [0.5], [0.0], [1.0])
+    assert abs(u[0]) <= 1e-15 and abs(v[0] - 0.5 / (2 * np.pi * 3.75)) <= 1e-15
+    u, v = oracle.evaluate(spec("periodic", box_length=2.0), [0.0, 1e-10], [0.0, 0.0],
+                           [1.0, -1.0])
+    assert abs(v[0] / ((0.5 / np.pi) / 1e-10) - 1) <= 1e-12
+    u, v = oracle.evaluate(spec("disk"), [0.2, 1e-155], [0.0, 0.0], [1.0, 1.0])
+    assert abs(v[0] - (0.5 / np.pi) * (5 + 0.2 / 0.96)) <= 1e-14
+    assert abs(v[1] - (0.5 / np.pi) * (-5 + 0.2)) <= 1e-14
+    with pytest.raises(RuntimeError, match="coincident periodic vortices"):
+        oracle.evaluate(spec("periodic", box_length=2.0), [0.0, 0.0], [0.0, 0.0], [1.0, -1.0])
+    with pytest.raises(ValueError):
+        oracle.check_kernel(spec("periodic", image_layers=65))
+
+
+def test_oracle_rk4_pair_rotation(oracle):
+    """src/tests.cpp:42-51."""
+    x, y, g = np.array([-1.0, 1.0]), np.array([0.0, 0.0]), np.array([1.0, 1.0])
+    it = oracle.integrator(2)
+    for _ in range(1000):
+        it.rk4_step(spec("infinite"), x, y, g, 1e-3)
+    a = 1 / (4 * np.pi)
+    assert abs(x[1] - np.cos(a)) <= 2e-12 and abs(y[1] - np.sin(a)) <= 2e-12
+
+
+# ------------------------------------------------------------------ brute force (mpmath)
+
+
+def _mp_infinite(x, y, g, eps2=0):
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    n = len(x)
+    out = []
+    for i in range(n):
+        u = v = mp.mpf(0)
+        for j in range(n):
+            if i == j:
+                continue
+            dx, dy = mp.mpf(x[i]) - x[j], mp.mpf(y[i]) - y[j]
+            c = g[j] / (2 * mp.pi * (dx * dx + dy * dy + eps2))
+            u -= c * dy
+            v += c * dx
+        out.append((float(u), float(v)))
+    return np.array(out)
+
+
+def _mp_disk(x, y, g, R):
+    """Direct sum plus the circle-theorem image vortex of strength -G at R^2 x/|x|^2."""
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    n = len(x)
+    out = []
+    for i in range(n):
+        u = v = mp.mpf(0)
+        for j in range(n):
+            xj, yj = mp.mpf(x[j]), mp.mpf(y[j])
+            if i != j:
+                dx, dy = mp.mpf(x[i]) - xj, mp.mpf(y[i]) - yj
+                c = g[j] / (2 * mp.pi * (dx * dx + dy * dy))
+                u -= c * dy
+                v += c * dx
+            r2 = xj * xj + yj * yj
+            xi_, yi_ = R * R * xj / r2, R * R * yj / r2
+            dx, dy = mp.mpf(x[i]) - xi_, mp.mpf(y[i]) - yi_
+            c = -g[j] / (2 * mp.pi * (dx * dx + dy * dy))
+            u -= c * dy
+            v += c * dx
+        out.append((float(u), float(v)))
+    return np.array(out)
+
+
+def _mp_periodic(x, y, g, L, layers):
+    """Rows of point vortices summed directly: for row m (x shifted by mL from the minimum
+    image of the pair), the sum over
+    y-copies n of the Biot-Savart u, paired (n, -n) so that it converges absolutely; v from
+    the complementary orientation (columns)."""
+    import mpmath as mp
+
+    mp.mp.dps = 30
+    n = len(x)
+    out = []
+    for i in range(n):
+        u = v = mp.mpf(0)
+        for j in range(n):
+            # The reference reduces each difference with remainder() before the row sums.
+            dx0 = mp.mpf(math.remainder(x[i] - x[j], L))
+            dy0 = mp.mpf(math.remainder(y[i] - y[j], L))
+            for m in range(-layers, layers + 1):
+                # u: row of vortices along y at x offset m L.
+                dx = dx0 - m * L
+
+                def fu(k, dx=dx):
+                    k = int(k)
+                    s = mp.mpf(0)
+                    for kk in ((k, -k) if k else (0,)):
+                        dy = dy0 - kk * L
+                        r2 = dx * dx + dy * dy
+                        if r2 == 0:
+                            continue
+                        s += -dy / r2
+                    return s
+
+                dy = dy0 - m * L
+
+                def fv(k, dy=dy):
+                    k = int(k)
+                    s = mp.mpf(0)
+                    for kk in ((k, -k) if k else (0,)):
+                        ddx = dx0 - kk * L
+                        r2 = ddx * ddx + dy * dy
+                        if r2 == 0:
+                            continue
+                        s += ddx / r2
+                    return s
+
+                u += g[j] / (2 * mp.pi) * mp.nsum(fu, [0, mp.inf])
+                v += g[j] / (2 * mp.pi) * mp.nsum(fv, [0, mp.inf])
+        out.append((float(u), float(v)))
+    return np.array(out)
+
+
+def test_oracle_against_brute_force(oracle):
+    rng = np.random.default_rng(3)
+    x, y = rng.uniform(-0.8, 0.8, 5), rng.uniform(-0.8, 0.8, 5)
+    g = np.array([1.0, -2.0, 0.5, 1.5, -1.0])
+    u, v = oracle.evaluate(spec("infinite", core_radius=0.1), x, y, g)
+    ref = _mp_infinite(x, y, g, eps2=0.01)
+    assert normwise(u, v, ref[:, 0], ref[:, 1]) <= 1e-15
+    r = np.sqrt(rng.uniform(0, 1, 5)) * 0.9
+    th = rng.uniform(0, 2 * np.pi, 5)
+    xd, yd = r * np.cos(th), r * np.sin(th)
+    u, v = oracle.evaluate(spec("disk", radius=1.0), xd, yd, g)
+    ref = _mp_disk(xd, yd, g, 1.0)
+    assert normwise(u, v, ref[:, 0], ref[:, 1]) <= 1e-14
+
+
+def test_oracle_periodic_rows_against_brute_force(oracle):
+    """With L = 1 layer the oracle's sum is 3 rows (and 3 columns) of point vortices; summed
+    directly here from Biot-Savart."""
+    x = np.array([0.1, -0.3, 0.45, -0.7])
+    y = np.array([0.2, 0.5, -0.6, -0.1])
+    g = np.array([1.0, -1.0, 0.5, -0.5])
+    u, v = oracle.evaluate(spec("periodic", box_length=2.0, image_layers=1), x, y, g)
+    ref = _mp_periodic(x, y, g, 2.0, 1)
+    assert normwise(u, v, ref[:, 0], ref[:, 1]) <= 1e-13
+
+
+def test_planted_mistakes_fail_the_checks(oracle):
+    """Plausible mistakes (sign, dropped image, transposed operands), planted in a copy of the
+    oracle's formulas on the CPU, are caught by the golden vectors."""
+    d = golden("disk")
+    x, y, g = d["n258_x"], d["n258_y"], d["n258_g"]
+    good_u, good_v = d["n258_u"], d["n258_v"]
+    mu, mv = _mp_disk(x[:6], y[:6], g[:6], 1.0).T
+    u, v = oracle.evaluate(spec("disk"), x[:6], y[:6], g[:6])
+    assert normwise(u, v, mu, mv) <= 1e-13
+    # sign flip of the image term
+    bad = _mp_infinite(x[:6], y[:6], g[:6])
+    assert normwise(bad[:, 0], bad[:, 1], mu, mv) > 1e-3
+    # transposed components
+    assert normwise(good_v, good_u, good_u, good_v) > 1e-1
+
+
+# ------------------------------------------------------------------ reference build
+
+
+@pytest.mark.parametrize("geometry", GEOMETRIES)
+def test_oracle_bitwise_equals_reference_build(oracle, reference, geometry):
+    pop = {"infinite": wl.infinite(700, 5), "periodic": wl.periodic(300, 5, box_length=2.0),
+           "disk": wl.disk(700, 5)}[geometry]
+    k = spec(geometry, box_length=2.0)
+    u, v = oracle.evaluate(k, pop.x, pop.y, pop.circulation)
+    ur, vr = reference.evaluate(k, pop.x, pop.y, pop.circulation)
+    assert np.array_equal(u, ur) and np.array_equal(v, vr)
+    kc = spec(geometry, core_radius=0.03, box_length=2.0)
+    assert np.array_equal(oracle.invariants(kc, pop.x, pop.y, pop.circulation),
+                          reference.invariants(kc, pop.x, pop.y, pop.circulation))
