@@ -137,6 +137,9 @@ def main():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the per-geometry table")
+    ap.add_argument("--only", choices=["fp64", "geometries", "copy"], default=None,
+                    help="profiling runs: only the FP64 RK4 run, only the per-geometry table, "
+                         "or only the 256 MiB copy-kernel calibration")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this backend has no multi-GPU path "
@@ -148,6 +151,13 @@ def main():
 
     dt = 1e-4
     n = args.n
+    if args.only == "copy":
+        ms = pv.backend.calibration_copy(256 << 20, 20)
+        log(f"[bench] copy kernel 256 MiB: {ms:.4f} ms ({2 * (256 << 20) / ms / 1e6:.1f} GB/s)")
+        return
+    if args.only == "geometries":
+        print(json.dumps(per_geometry(pv, n)), flush=True)
+        return
     pop = wl.disk(n, 2)
     pairs_per_step = 4 * n * (n - 1)
     log(f"[bench] disk N={n} RK4 dt={dt} steps={args.steps} warmup={args.warmup}")
@@ -155,23 +165,26 @@ def main():
     k64, wall64, kms64, launches64 = run_rk4(pv, pop, "fp64", args.steps, args.warmup, dt)
     value = args.steps * pairs_per_step / wall64
     log(f"[bench] fp64: {wall64:.3f} s, {value:.4e} pairs/s, kernel {kms64 / launches64:.3f} ms")
+    if args.only == "fp64":
+        return
     kmx, wallmx, kmsmx, launchesmx = run_rk4(pv, pop, "mixed", args.steps, args.warmup, dt)
     value_mx = args.steps * pairs_per_step / wallmx
     log(f"[bench] mixed: {wallmx:.3f} s, {value_mx:.4e} pairs/s")
 
     # Velocity error of the mixed kernel against the FP64 kernel on the mixed run's final state,
     # and on the initial state.
-    errs = []
+    errs = {}
     fp64_eval = pv.HipKernel(pv.SimParams(boundaryCondition="disk", diskRadius=1.0))
-    for state in ("initial", "final"):
+    for state in ("final", "initial"):
         s = pv.VortexSystem(pop.x.copy(), pop.y.copy(), pop.circulation)
         if state == "final":
             kmx.downloadDeviceState(s)
-        kmx.uploadDeviceState(s) if state == "initial" else None
+        else:
+            kmx.uploadDeviceState(s)
         fp64_eval.uploadDeviceState(s)
         ref = fp64_eval.evaluateDeviceState()
         got = kmx.evaluateDeviceState()
-        errs.append(normwise(got.x, got.y, ref.x, ref.y))
+        errs[state] = normwise(got.x, got.y, ref.x, ref.y)
     log(f"[bench] mixed normwise error vs fp64: {errs}")
 
     avg_ms = kms64 / launches64
@@ -213,8 +226,8 @@ def main():
             "unit": "pair-interactions/s",
             "dtype": "f64-differences/f32-terms/f64-accumulation",
             "speedup_vs_fp64": value_mx / value,
-            "normwise_err_vs_fp64_initial": errs[0],
-            "normwise_err_vs_fp64_final": errs[1],
+            "normwise_err_vs_fp64_initial": errs["initial"],
+            "normwise_err_vs_fp64_final": errs["final"],
             "avg_launch_ms": kmsmx / launchesmx,
         },
     }

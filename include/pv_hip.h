@@ -171,6 +171,11 @@ pv_status pv_set_launch(pv_handle *h, int targets_per_thread, int source_splits)
 pv_status pv_profile_begin(pv_handle *h);
 pv_status pv_profile_end(pv_handle *h, double *kernel_ms, int *launches);
 
+/* Measurement helper (no reference counterpart): copies `bytes` bytes between two device
+ * buffers with a plain grid-stride copy kernel, `repeats` times, and returns the average
+ * kernel time.  Used to calibrate the HBM-traffic counters (FETCH_SIZE, WRITE_SIZE). */
+pv_status pv_calibration_copy(int device, size_t bytes, int repeats, double *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,7 @@ _SIGNATURES = {
     "pv_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "pv_profile_begin": (ctypes.c_int, [_vp]),
     "pv_profile_end": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int)]),
+    "pv_calibration_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp]),
 }
 
 _lib = None
@@ -391,6 +392,14 @@ def backendRuntimeDetails(device: int = 0) -> str:
     buf = ctypes.create_string_buffer(1024)
     _check(lib.pv_runtime_details(int(device), buf, len(buf)))
     return buf.value.decode()
+
+
+def calibration_copy(nbytes: int, repeats: int = 10, device: int = 0) -> float:
+    """Average milliseconds of a plain device copy kernel of nbytes (counter calibration)."""
+    lib = load_library()
+    ms = ctypes.c_double(0.0)
+    _check(lib.pv_calibration_copy(int(device), int(nbytes), int(repeats), ctypes.byref(ms)))
+    return ms.value
 
 
 def declared_symbols(header: Path = HEADER_PATH):

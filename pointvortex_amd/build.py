@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 COMMON = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast", f"-I{INCLUDE}", f"-I{CSRC}",
+    "-ffp-contract=fast-honor-pragmas", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
 SOURCES = ["pv_kernels.hip", "pv_api.cpp"]

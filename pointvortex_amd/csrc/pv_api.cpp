@@ -1051,4 +1051,50 @@ pv_status pv_profile_end(pv_handle *h, double *kernel_ms, int *launches) {
     });
 }
 
+pv_status pv_calibration_copy(int device, size_t bytes, int repeats, double *ms_per_copy) {
+    return guard([&] {
+        if (bytes < 32 || repeats <= 0)
+            raise(PV_ERR_INVALID_ARGUMENT, "copy size must be >= 32 bytes and repeats > 0");
+        DeviceGuard dg(device);
+        const size_t count = bytes / 32;
+        double4 *src = nullptr, *dst = nullptr;
+        hipStream_t stream = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (src)
+                (void)hipFree(src);
+            if (dst)
+                (void)hipFree(dst);
+            if (e0)
+                (void)hipEventDestroy(e0);
+            if (e1)
+                (void)hipEventDestroy(e1);
+            if (stream)
+                (void)hipStreamDestroy(stream);
+        };
+        try {
+            hip_check(hipMalloc((void **)&src, count * 32), "hipMalloc(copy source)");
+            hip_check(hipMalloc((void **)&dst, count * 32), "hipMalloc(copy destination)");
+            hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "create HIP stream");
+            hip_check(hipMemsetAsync(src, 0, count * 32, stream), "clear copy source");
+            hip_check(hipEventCreate(&e0), "create HIP event");
+            hip_check(hipEventCreate(&e1), "create HIP event");
+            pv::launch_copy(src, dst, count, stream);
+            hip_check(hipEventRecord(e0, stream), "record HIP event");
+            for (int r = 0; r < repeats; ++r)
+                pv::launch_copy(src, dst, count, stream);
+            hip_check(hipEventRecord(e1, stream), "record HIP event");
+            hip_check(hipEventSynchronize(e1), "wait for HIP event");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "read HIP event time");
+            if (ms_per_copy)
+                *ms_per_copy = ms / repeats;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
+
 } // extern "C"

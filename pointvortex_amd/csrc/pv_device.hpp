@@ -8,6 +8,9 @@
 //   pair()          one source's contribution to one target (j != i guaranteed);
 //   pair_checked()  the same, for the few source tiles that contain the target itself;
 //   fold()          called after each LDS tile (mixed: FP32 tile sum -> FP64 TwoSum).
+//   kBatched        FP64 policies split pair() into prep() (the pair's denominators) and
+//                   apply() (given their inverses), so that the kernel inverts all the
+//                   denominators of a group of pairs with one rcp64 (TreeInverse).
 //
 // The pair formulas restate src/compute.cpp:117-126 (infinite), 223-249 (periodic) and
 // 306-329 (disk).  The periodic image sum uses the exact Fourier form of the reference's
@@ -51,6 +54,33 @@ __device__ __forceinline__ double rcp64(double a) {
 }
 
 __device__ __forceinline__ float rcp32(float a) { return __builtin_amdgcn_rcpf(a); }
+
+// Reciprocals of K positive values with one rcp64 (K a power of two): a product tree, one
+// refined reciprocal of the root, then back-multiplication down the tree; about 2 log2(K) ulps.
+// Returns false (inverses then unusable) when the root leaves [2^-960, 2^960], where a partial
+// product could have lost bits; callers then fall back to one rcp64 per value.
+template <int K> struct TreeInverse {
+    __device__ static __forceinline__ bool run(const double *d, double *inv) {
+        static_assert((K & (K - 1)) == 0 && K >= 2, "K must be a power of two");
+        double lo[K / 2], lo_inv[K / 2];
+#pragma unroll
+        for (int k = 0; k < K / 2; ++k)
+            lo[k] = d[2 * k] * d[2 * k + 1];
+        const bool ok = TreeInverse<K / 2>::run(lo, lo_inv);
+#pragma unroll
+        for (int k = 0; k < K / 2; ++k) {
+            inv[2 * k] = lo_inv[k] * d[2 * k + 1];
+            inv[2 * k + 1] = lo_inv[k] * d[2 * k];
+        }
+        return ok;
+    }
+};
+template <> struct TreeInverse<1> {
+    __device__ static __forceinline__ bool run(const double *d, double *inv) {
+        inv[0] = rcp64(d[0]);
+        return d[0] >= 0x1p-960 && d[0] <= 0x1p960;
+    }
+};
 
 // Error-free addition (Knuth's TwoSum): s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
@@ -170,6 +200,7 @@ struct InfF64 {
         double u, v;
     };
     static constexpr int kFoldEvery = 0;
+    static constexpr bool kBatched = true;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, kInvTwoPi * g, 0.0};
     }
@@ -196,6 +227,22 @@ struct InfF64 {
         u = a.u;
         v = a.v;
     }
+    // Batched form: prep() returns the pair's denominators, apply() consumes their inverses.
+    static constexpr int kDenoms = 1;
+    struct Tmp {
+        double dx, dy;
+    };
+    __device__ static void prep(const GeoParams &p, const Tgt &t, const Src &s, Tmp &m,
+                                double *d) {
+        m.dx = t.x - s.x;
+        m.dy = t.y - s.y;
+        d[0] = __builtin_fma(m.dx, m.dx, __builtin_fma(m.dy, m.dy, p.eps2));
+    }
+    __device__ static void apply(const Src &s, const Tmp &m, const double *inv, Acc &a) {
+        const double c = s.g * inv[0];
+        a.u = __builtin_fma(-c, m.dy, a.u);
+        a.v = __builtin_fma(c, m.dx, a.v);
+    }
 };
 
 // Mixed: FP64 difference, FP32 1/r^2 and products, FP32 tile sums folded into FP64.
@@ -213,6 +260,7 @@ struct InfMixed {
         Comp u, v;
     };
     static constexpr int kFoldEvery = 1;
+    static constexpr bool kBatched = false;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
     }
@@ -265,6 +313,7 @@ struct DiskF64 {
         double u, v;
     };
     static constexpr int kFoldEvery = 0;
+    static constexpr bool kBatched = true;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, kInvTwoPi * g, x * p.invR, y * p.invR, kInvTwoPi * g * p.invR};
     }
@@ -304,6 +353,28 @@ struct DiskF64 {
     __device__ static void result(const Acc &a, double &u, double &v) {
         u = a.u;
         v = a.v;
+    }    static constexpr int kDenoms = 2;
+    struct Tmp {
+        double dx, dy, a, b;
+    };
+    __device__ static void prep(const GeoParams &, const Tgt &t, const Src &s, Tmp &m,
+                                double *d) {
+        m.dx = t.x - s.x;
+        m.dy = t.y - s.y;
+        d[0] = __builtin_fma(m.dx, m.dx, m.dy * m.dy);
+        m.a = 1.0 - __builtin_fma(t.tx, s.sx, t.ty * s.sy);
+        m.b = __builtin_fma(t.ty, s.sx, -(t.tx * s.sy));
+        d[1] = __builtin_fma(m.a, m.a, m.b * m.b);
+    }
+    __device__ static void apply(const Src &s, const Tmp &m, const double *inv, Acc &a) {
+        const double c = s.g * inv[0];
+        a.u = __builtin_fma(-c, m.dy, a.u);
+        a.v = __builtin_fma(c, m.dx, a.v);
+        const double k = s.gr * inv[1];
+        const double pu = __builtin_fma(m.a, s.sy, -(m.b * s.sx));
+        const double pv = __builtin_fma(m.a, s.sx, m.b * s.sy);
+        a.u = __builtin_fma(-k, pu, a.u);
+        a.v = __builtin_fma(k, pv, a.v);
     }
 };
 
@@ -324,6 +395,7 @@ struct DiskMixed {
         Comp u, v;
     };
     static constexpr int kFoldEvery = 1;
+    static constexpr bool kBatched = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), (float)(x * p.invR), (float)(y * p.invR),
                 (float)(kInvTwoPi * g * p.invR)};
@@ -400,6 +472,7 @@ struct PerF64 {
         double u, v;
     };
     static constexpr int kFoldEvery = 0;
+    static constexpr bool kBatched = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, p.scale * g, 0.0};
     }
@@ -485,6 +558,7 @@ struct PerMixed {
         Comp u, v;
     };
     static constexpr int kFoldEvery = 1;
+    static constexpr bool kBatched = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
     }

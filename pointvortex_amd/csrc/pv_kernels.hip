@@ -144,6 +144,31 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
         const bool diag = base < t0 + TT && t0 < base + cnt;
         if (!diag) {
             int k = 0;
+            if constexpr (P::kBatched) {
+                // Two sources x TPT targets: all their denominators share one rcp64.
+                constexpr int ND = P::kDenoms, K = 2 * TPT * ND;
+                for (; k + 2 <= cnt; k += 2) {
+                    const typename P::Src s2[2] = {sh[k], sh[k + 1]};
+                    typename P::Tmp m[2][TPT];
+                    double d[K], inv[K];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t)
+                            P::prep(gp, tg[t], s2[q], m[q][t], &d[(q * TPT + t) * ND]);
+                    const bool ok = TreeInverse<K>::run(d, inv);
+                    if (!__all(ok)) {
+#pragma unroll
+                        for (int e = 0; e < K; ++e)
+                            inv[e] = rcp64(d[e]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t)
+                            P::apply(s2[q], m[q][t], &inv[(q * TPT + t) * ND], acc[t]);
+                }
+            }
             for (; k + 2 <= cnt; k += 2) {
                 const typename P::Src s_a = sh[k];
                 const typename P::Src s_b = sh[k + 1];
@@ -262,6 +287,7 @@ __global__ void make_stage_kernel(StageOp op, int n, const DevStatus *st, unsign
 // coincident pair apart from an overflow.
 __global__ void classify_kernel(GeoParams gp, const double *x, const double *y, int n,
                                 int tbegin, int tend, DevStatus *st) {
+#pragma clang fp contract(off)
     const int i = tbegin + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= tend)
         return;
@@ -270,18 +296,18 @@ __global__ void classify_kernel(GeoParams gp, const double *x, const double *y, 
         if (j == i)
             continue;
         if (gp.geometry == 1) {
-            const double sdx = __dmul_rn(gp.wave, remainder(x[i] - x[j], gp.L));
-            const double sdy = __dmul_rn(gp.wave, remainder(y[i] - y[j], gp.L));
+            const double sdx = gp.wave * remainder(x[i] - x[j], gp.L);
+            const double sdy = gp.wave * remainder(y[i] - y[j], gp.L);
             const double shx = sinh(0.5 * sdx), shy = sinh(0.5 * sdy);
             const double snx = sin(0.5 * sdx), sny = sin(0.5 * sdy);
-            const double du = 2.0 * __dadd_rn(__dmul_rn(shx, shx), __dmul_rn(sny, sny));
-            const double dv = 2.0 * __dadd_rn(__dmul_rn(shy, shy), __dmul_rn(snx, snx));
+            const double du = 2.0 * (shx * shx + sny * sny);
+            const double dv = 2.0 * (shy * shy + snx * snx);
             hit = du == 0.0 || dv == 0.0;
         } else {
-            const double dx = __dsub_rn(x[i], x[j]), dy = __dsub_rn(y[i], y[j]);
-            double d = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+            const double dx = x[i] - x[j], dy = y[i] - y[j];
+            double d = dx * dx + dy * dy;
             if (gp.geometry == 0)
-                d = __dadd_rn(d, gp.eps2);
+                d = d + gp.eps2;
             hit = d == 0.0;
         }
     }
@@ -434,11 +460,12 @@ __global__ __launch_bounds__(kReduceBlock) void energy_kernel(GeoParams gp, cons
 }
 
 // Close opposite-sign pairs (src/dipole.cpp:90-118), with the reference's non-fused
-// arithmetic so that candidate sets and their d^2 match it bit for bit.
+// arithmetic (contraction off) so that candidate sets and their d^2 match it bit for bit.
 __global__ __launch_bounds__(kReduceBlock) void dipole_kernel(
     GeoParams gp, int periodic, int disk, double thr2, const double *x, const double *y,
     const double *g, int n, unsigned long long *first, unsigned long long *second, double *d2,
     unsigned long long cap, DevStatus *st) {
+#pragma clang fp contract(off)
     __shared__ double sx[kReduceBlock], sy[kReduceBlock], sg[kReduceBlock];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
@@ -463,12 +490,12 @@ __global__ __launch_bounds__(kReduceBlock) void dipole_kernel(
             const double gj = sg[k];
             if (gj == 0.0 || signbit(gi) == signbit(gj))
                 continue;
-            double dx = __dsub_rn(xi, sx[k]), dy = __dsub_rn(yi, sy[k]);
+            double dx = xi - sx[k], dy = yi - sy[k];
             if (periodic) {
                 dx = remainder_fast(dx, gp.L, gp.invL);
                 dy = remainder_fast(dy, gp.L, gp.invL);
             }
-            const double dd = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+            const double dd = dx * dx + dy * dy;
             if (dd < thr2) {
                 const unsigned long long slot = atomicAdd(&st->count, 1ull);
                 if (slot < cap) {
@@ -480,10 +507,10 @@ __global__ __launch_bounds__(kReduceBlock) void dipole_kernel(
         }
     }
     if (active && disk) {
-        const double ri2 = __dadd_rn(__dmul_rn(xi, xi), __dmul_rn(yi, yi));
+        const double ri2 = xi * xi + yi * yi;
         if (ri2 > 0.0) {
             const double idist = (gp.R2 - ri2) / sqrt(ri2);
-            const double dd = __dmul_rn(idist, idist);
+            const double dd = idist * idist;
             if (dd < thr2) {
                 const unsigned long long slot = atomicAdd(&st->count, 1ull);
                 if (slot < cap) {
@@ -494,6 +521,14 @@ __global__ __launch_bounds__(kReduceBlock) void dipole_kernel(
             }
         }
     }
+}
+
+// Plain streaming copy (traffic-counter calibration).
+__global__ void copy_kernel(const double4 *__restrict__ src, double4 *__restrict__ dst,
+                            size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // ------------------------------------------------------------------ launchers
@@ -573,6 +608,10 @@ void launch_dipoles(const GeoParams &gp, bool periodic, bool disk, double thr2, 
     hipLaunchKernelGGL(dipole_kernel, dim3((n + kReduceBlock - 1) / kReduceBlock),
                        dim3(kReduceBlock), 0, s, gp, (int)periodic, (int)disk, thr2, x, y, g, n,
                        first, second, d2, cap, st);
+}
+
+void launch_copy(const double4 *src, double4 *dst, size_t count, hipStream_t s) {
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s, src, dst, count);
 }
 
 } // namespace pv

@@ -74,6 +74,8 @@ void launch_dipoles(const GeoParams &gp, bool periodic, bool disk, double thr2, 
                     unsigned long long *second, double *d2, unsigned long long cap,
                     DevStatus *st, hipStream_t s);
 
+void launch_copy(const double4 *src, double4 *dst, size_t count, hipStream_t s);
+
 constexpr int kReduceBlock = 256;
 
 } // namespace pv
