@@ -176,6 +176,64 @@ pv_status pv_profile_end(pv_handle *h, double *kernel_ms, int *launches);
  * kernel time.  Used to calibrate the HBM-traffic counters (FETCH_SIZE, WRITE_SIZE). */
 pv_status pv_calibration_copy(int device, size_t bytes, int repeats, double *ms_per_copy);
 
+/* ------------------------------------------------------------------------------------
+ * Ensembles (no reference counterpart; each instance is what the reference runs as one
+ * process).  Instances of uneven N and any geometry share flat device arrays and advance
+ * together, one persistent launch per RK stage.
+ * ---------------------------------------------------------------------------------- */
+
+typedef struct pv_batch pv_batch;
+
+/* Geometry of one instance (the SimParams fields of pv_config). */
+typedef struct pv_instance_config {
+    int geometry;
+    int image_layers;
+    double core_radius;
+    double box_length;
+    double disk_radius;
+} pv_instance_config;
+
+/* Device-side state of one adaptive instance, as read back by pv_batch_poll. */
+typedef struct pv_slot_state {
+    int slot;
+    int state;             /* 0 running, 1 finished, 2 failed, 3 idle */
+    double t, dt, last_error;
+    unsigned accepted, rejected;
+    unsigned failure;      /* failure kind bits; see pv_batch_failure_message */
+    size_t n;
+} pv_slot_state;
+
+/* Room for `capacity` vortices and `max_instances` instances; 208 bytes of HBM per vortex. */
+pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max_instances,
+                          pv_batch **out);
+void pv_batch_destroy(pv_batch *b);
+pv_status pv_batch_set_precision(pv_batch *b, int precision);
+size_t pv_batch_bytes_per_vortex(void);
+
+/* Fixed-step ensembles: replace the contents with `ninst` instances laid out one after the
+ * other in x, y, gamma (counts[k] vortices each), advance them with RK4 (optionally as
+ * replays of a captured hipGraph), read positions back, or evaluate velocities. */
+pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const size_t *counts,
+                        size_t ninst, const double *x, const double *y, const double *gamma);
+pv_status pv_batch_rk4(pv_batch *b, double dt, int steps, int use_graph);
+pv_status pv_batch_read(pv_batch *b, double *x, double *y, size_t total);
+pv_status pv_batch_evaluate(pv_batch *b, double *u, double *v, size_t total);
+
+/* Adaptive ensembles with admission while others run (serving).  pv_batch_admit places one
+ * run (DOPRI5 from t=0 to t_end) if its vortices fit; *slot is -1 when they do not.
+ * pv_batch_attempt performs one DOPRI5 attempt for every running instance with the
+ * step-size controllers on the device; pv_batch_poll reads every occupied slot's state;
+ * pv_batch_fetch copies a slot's positions; pv_batch_release frees the slot. */
+pv_status pv_batch_admit(pv_batch *b, const pv_instance_config *config, const double *x,
+                         const double *y, const double *gamma, size_t n, double t_end,
+                         double dt0, double atol, double rtol, double min_dt, double max_dt,
+                         int *slot);
+pv_status pv_batch_attempt(pv_batch *b);
+pv_status pv_batch_poll(pv_batch *b, pv_slot_state *states, size_t cap, size_t *count);
+pv_status pv_batch_fetch(pv_batch *b, int slot, double *x, double *y, size_t n);
+pv_status pv_batch_release(pv_batch *b, int slot);
+const char *pv_batch_failure_message(unsigned failure);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,30 @@ class PvInvariants(ctypes.Structure):
     ]
 
 
+class PvInstanceConfig(ctypes.Structure):
+    _fields_ = [
+        ("geometry", ctypes.c_int),
+        ("image_layers", ctypes.c_int),
+        ("core_radius", ctypes.c_double),
+        ("box_length", ctypes.c_double),
+        ("disk_radius", ctypes.c_double),
+    ]
+
+
+class PvSlotState(ctypes.Structure):
+    _fields_ = [
+        ("slot", ctypes.c_int),
+        ("state", ctypes.c_int),
+        ("t", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("last_error", ctypes.c_double),
+        ("accepted", ctypes.c_uint),
+        ("rejected", ctypes.c_uint),
+        ("failure", ctypes.c_uint),
+        ("n", ctypes.c_size_t),
+    ]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _szp = ctypes.POINTER(ctypes.c_size_t)
@@ -115,6 +139,24 @@ _SIGNATURES = {
     "pv_profile_begin": (ctypes.c_int, [_vp]),
     "pv_profile_end": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "pv_calibration_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp]),
+    "pv_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                       ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "pv_batch_destroy": (None, [_vp]),
+    "pv_batch_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pv_batch_bytes_per_vortex": (ctypes.c_size_t, []),
+    "pv_batch_load": (ctypes.c_int, [_vp, ctypes.POINTER(PvInstanceConfig), _szp,
+                                     ctypes.c_size_t, _dp, _dp, _dp]),
+    "pv_batch_rk4": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "pv_batch_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_size_t]),
+    "pv_batch_evaluate": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_size_t]),
+    "pv_batch_admit": (ctypes.c_int, [_vp, ctypes.POINTER(PvInstanceConfig), _dp, _dp, _dp,
+                                      ctypes.c_size_t] + [ctypes.c_double] * 6 +
+                       [ctypes.POINTER(ctypes.c_int)]),
+    "pv_batch_attempt": (ctypes.c_int, [_vp]),
+    "pv_batch_poll": (ctypes.c_int, [_vp, ctypes.POINTER(PvSlotState), ctypes.c_size_t, _szp]),
+    "pv_batch_fetch": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_size_t]),
+    "pv_batch_release": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pv_batch_failure_message": (ctypes.c_char_p, [ctypes.c_uint]),
 }
 
 _lib = None
@@ -407,5 +449,5 @@ def declared_symbols(header: Path = HEADER_PATH):
     import re
 
     text = header.read_text()
-    return sorted(set(re.findall(r"^\s*(?:pv_status|void|const char \*)\s*(pv_\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:pv_status|void|const char \*|size_t)\s*(pv_\w+)\s*\(",
                                  text, re.M)))

@@ -1,0 +1,71 @@
+// pv_batch.hpp -- batched ensembles: many independent vortex systems of uneven N advanced
+// together, one persistent launch per RK stage.
+#pragma once
+#include "pv_kernels.hpp"
+
+namespace pv {
+
+// One instance of the ensemble: its vortices are [offset, offset + n) of the flat SoA arrays.
+struct Instance {
+    GeoParams gp;
+    int offset;
+    int n;
+};
+
+// One unit of work: 64 targets [tbegin, tend) (instance-local) of one instance against all
+// of that instance's sources.  Items are sorted by decreasing cost on the host, and waves take
+// them from a shared counter, so the largest items start first (longest-processing-time
+// order); every target's sum is still computed by one wave in a fixed order.
+struct WorkItem {
+    int instance;
+    int tbegin, tend;
+    int pad;
+};
+
+// Per-instance adaptive DOPRI5 state (src/timestep.cpp:62-119 and the step landing of
+// src/main.cpp:312-314), resident on the device.
+struct InstanceStep {
+    double t, dt, sugg, t_end;        // time, trial step, carried suggestion, horizon
+    double atol, rtol, min_dt, max_dt;
+    double last_error;
+    unsigned long long err_bits;      // max-norm error of the current attempt
+    int state;                        // kRunning, kFinished, kFailed, kIdle
+    int fsal;                         // k0 holds f(x) at the current state
+    int accept_now;                   // set by the controller for the accept pass
+    unsigned accepted, rejected_total, rejected_now;
+    unsigned failure;                 // FailureKind bits (kFailRejected: too many rejections)
+    int pad;
+};
+
+enum InstanceState : int { kRunning = 0, kFinished = 1, kFailed = 2, kIdle = 3 };
+constexpr unsigned kFailRejected = 32u;
+
+enum BatchPass : int {
+    kPassAll = 0,        // every item
+    kPassNeedK0 = 1,     // only instances whose FSAL derivative is missing (adaptive)
+};
+
+struct BatchLaunch {
+    const Instance *inst;
+    const WorkItem *items;
+    int nitems;
+    unsigned *counter;         // next item; zeroed before every launch
+    const double *x, *y, *g;   // positions evaluated (flat) and circulations
+    DevStatus *st;
+    unsigned epoch;
+    StageOp op;
+    InstanceStep *steps;       // adaptive mode: per-instance dt, error and failure sink
+    int pass;
+};
+
+// Launchers (pv_batch.hip).
+void launch_batch(int precision, bool adaptive, const BatchLaunch &a, int blocks, hipStream_t s);
+void launch_batch_reset(unsigned *counter, hipStream_t s);
+void launch_batch_stage1(const StageOp &op, const int *owner, const InstanceStep *steps, int n,
+                         hipStream_t s);
+void launch_batch_control(InstanceStep *steps, int ninst, hipStream_t s);
+void launch_batch_accept(const int *owner, const InstanceStep *steps, double *x, double *y,
+                         const double *xn, const double *yn, double *k0x, double *k0y,
+                         const double *k6x, const double *k6y, int n, hipStream_t s);
+
+} // namespace pv

@@ -1,0 +1,755 @@
+// pv_batch_api.cpp -- C-ABI of the ensemble engine (pv_batch_* in include/pv_hip.h).
+//
+// All instances share flat SoA device arrays (x, y, G, next positions, two stage position
+// buffers, seven stage velocities, u, v).  Fixed-step ensembles are laid out back to back by
+// pv_batch_load; adaptive runs (serving) take vortex ranges from a first-fit allocator and an
+// instance slot each, so runs can be admitted and released while others are in flight.
+#include "pv_batch.hpp"
+#include "pv_hip.h"
+#include "pv_host.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace pv::host;
+
+namespace {
+
+// src/timestep.h:8-15 and src/timestep.cpp:104-108.
+constexpr double kB5[7] = {35.0 / 384.0,     0.0, 500.0 / 1113.0, 125.0 / 192.0,
+                           -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
+constexpr double kB4[7] = {5179.0 / 57600.0, 0.0,           7571.0 / 16695.0, 393.0 / 640.0,
+                           -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0};
+constexpr double kDopriA[7][7] = {
+    {0},
+    {1.0 / 5.0},
+    {3.0 / 40.0, 9.0 / 40.0},
+    {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0},
+    {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0},
+    {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0},
+    {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+
+constexpr size_t kDoublesPerVortex = 25;  // x y G, next x y, 2 stage x y, 7 k x y, u v
+
+pv_config to_config(const pv_instance_config &c, int precision) {
+    pv_config p{};
+    p.geometry = c.geometry;
+    p.precision = precision;
+    p.core_radius = c.core_radius;
+    p.box_length_x = p.box_length_y = c.box_length;
+    p.image_layers = c.image_layers;
+    p.disk_radius = c.disk_radius;
+    return p;
+}
+
+// Relative cost of one pair by geometry (measured kernel rates at N = 64K), used only to
+// order work items.
+double pair_weight(int geometry, int precision) {
+    if (geometry == PV_PERIODIC)
+        return precision ? 4.0 : 12.0;
+    if (geometry == PV_DISK)
+        return precision ? 1.9 : 2.0;
+    return 1.0;
+}
+
+void validate_instance(const pv_instance_config &c, const double *x, const double *y,
+                       const double *g, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(g[i]))
+            raise(PV_ERR_INVALID_ARGUMENT, "vortex data must be finite");
+    if (c.geometry == PV_PERIODIC) {
+        double total = 0.0, absolute = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            total += g[i];
+            absolute += std::fabs(g[i]);
+        }
+        if (std::fabs(total) > 1e-12 * std::max(1.0, absolute))
+            raise(PV_ERR_INVALID_ARGUMENT, "periodic box requires zero total circulation");
+    } else if (c.geometry == PV_DISK) {
+        const double r2 = c.disk_radius * c.disk_radius;
+        for (size_t i = 0; i < n; ++i)
+            if (x[i] * x[i] + y[i] * y[i] >= r2)
+                raise(PV_ERR_INVALID_ARGUMENT, "vortex lies on or outside the disk");
+    }
+}
+
+} // namespace
+
+struct pv_batch {
+    int device = 0;
+    int precision = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    size_t capacity = 0, max_inst = 0;
+
+    double *x = nullptr, *y = nullptr, *g = nullptr, *xn = nullptr, *yn = nullptr;
+    double *xa = nullptr, *ya = nullptr, *xb = nullptr, *yb = nullptr;
+    double *kx[7] = {}, *ky[7] = {};
+    double *u = nullptr, *v = nullptr;
+    int *owner = nullptr;
+
+    pv::Instance *inst_d = nullptr;
+    std::vector<pv::Instance> inst_h;
+    pv::InstanceStep *steps_d = nullptr;
+    pv::InstanceStep *steps_h = nullptr;   // pinned mirror
+    pv::WorkItem *items_d = nullptr;
+    size_t items_cap = 0;
+    int nitems = 0;
+    unsigned *counter = nullptr;
+    pv::DevStatus *st = nullptr;
+    pv::DevStatus *st_host = nullptr;
+    unsigned epoch = 0;
+
+    // fixed-step layout
+    size_t total = 0;
+    size_t nfixed = 0;
+    std::map<std::pair<double, bool>, hipGraphExec_t> graphs;  // (dt, orientation) -> 2 steps
+
+    // adaptive layout
+    std::vector<char> used;
+    std::vector<char> in_items;                           // slots the work items cover
+    std::vector<std::pair<size_t, size_t>> free_ranges;   // [begin, end), sorted
+    bool items_dirty = true;
+    bool adaptive = false;
+
+    ~pv_batch() {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        for (auto &kv : graphs)
+            (void)hipGraphExecDestroy(kv.second);
+        dfree(x), dfree(y), dfree(g), dfree(xn), dfree(yn);
+        dfree(xa), dfree(ya), dfree(xb), dfree(yb);
+        for (int s = 0; s < 7; ++s)
+            dfree(kx[s]), dfree(ky[s]);
+        dfree(u), dfree(v), dfree(owner);
+        dfree(inst_d), dfree(steps_d), dfree(items_d), dfree(counter), dfree(st);
+        if (steps_h)
+            (void)hipHostFree(steps_h);
+        if (st_host)
+            (void)hipHostFree(st_host);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+pv_batch *checked(pv_batch *b) {
+    if (!b)
+        raise(PV_ERR_INVALID_ARGUMENT, "null ensemble");
+    return b;
+}
+
+void reset_status(pv_batch *b) {
+    pv::launch_reset_status(b->st, b->stream);
+    hip_check(hipGetLastError(), "launch HIP status reset");
+}
+
+int batch_blocks(const pv_batch *b) {
+    const int want = (b->nitems + 3) / 4;
+    return std::max(1, std::min(want, 8 * b->cu_count));
+}
+
+void launch(pv_batch *b, const double *px, const double *py, const pv::StageOp &op,
+            bool adaptive, int pass) {
+    pv::launch_batch_reset(b->counter, b->stream);
+    hip_check(hipGetLastError(), "launch HIP ensemble reset");
+    pv::BatchLaunch a{};
+    a.inst = b->inst_d;
+    a.items = b->items_d;
+    a.nitems = b->nitems;
+    a.counter = b->counter;
+    a.x = px, a.y = py, a.g = b->g;
+    a.st = b->st;
+    a.epoch = ++b->epoch;
+    a.op = op;
+    a.steps = adaptive ? b->steps_d : nullptr;
+    a.pass = pass;
+    pv::launch_batch(b->precision, adaptive, a, batch_blocks(b), b->stream);
+    hip_check(hipGetLastError(), "launch HIP ensemble kernel");
+}
+
+// Cost-sorted work items of the given instances (longest first).
+void build_items(pv_batch *b, const std::vector<int> &instances) {
+    struct Keyed {
+        double cost;
+        pv::WorkItem item;
+    };
+    std::vector<Keyed> v;
+    for (int k : instances) {
+        const int n = b->inst_h[k].n;
+        const double w = pair_weight(b->inst_h[k].gp.geometry, b->precision) * n;
+        for (int tb = 0; tb < n; tb += 64)
+            v.push_back({w, {k, tb, std::min(n, tb + 64), 0}});
+    }
+    std::stable_sort(v.begin(), v.end(),
+                     [](const Keyed &a, const Keyed &c) { return a.cost > c.cost; });
+    if (v.size() > b->items_cap) {
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        dfree(b->items_d);
+        b->items_cap = 0;
+        b->items_d = dalloc<pv::WorkItem>(v.size(), "hipMalloc(work items)");
+        b->items_cap = v.size();
+    }
+    std::vector<pv::WorkItem> items(v.size());
+    for (size_t i = 0; i < v.size(); ++i)
+        items[i] = v[i].item;
+    if (!items.empty()) {
+        hip_check(hipMemcpyAsync(b->items_d, items.data(), items.size() * sizeof(pv::WorkItem),
+                                 hipMemcpyHostToDevice, b->stream),
+                  "upload work items");
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    }
+    b->nitems = (int)items.size();
+}
+
+void read_status(pv_batch *b) {
+    hip_check(hipMemcpyAsync(b->st_host, b->st, sizeof(pv::DevStatus), hipMemcpyDeviceToHost,
+                             b->stream),
+              "read HIP status");
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+}
+
+void finish_fixed(pv_batch *b) {
+    read_status(b);
+    const unsigned f = b->st_host->flags;
+    if (!f)
+        return;
+    if (f & pv::kFailStateNonFinite)
+        raise(PV_ERR_INVALID_ARGUMENT, "ensemble: vortex data must be finite");
+    if (f & pv::kFailOutsideDisk)
+        raise(PV_ERR_INVALID_ARGUMENT, "ensemble: vortex lies on or outside the disk");
+    if (f & pv::kFailVelocity)
+        raise(PV_ERR_RUNTIME,
+              "ensemble: non-finite velocity (coincident vortices or overflow) in an instance");
+    raise(PV_ERR_RUNTIME, "ensemble: evaluation failed");
+}
+
+// One RK4 step of every instance from (x0, y0) into (x1, y1) (src/timestep.cpp:31-61).
+void enqueue_rk4(pv_batch *b, double dt, const double *x0, const double *y0, double *x1,
+                 double *y1) {
+    pv::StageOp op{};
+    op.dt = dt;
+    op.x0 = x0, op.y0 = y0;
+    for (int s = 0; s < 7; ++s)
+        op.kx[s] = b->kx[s], op.ky[s] = b->ky[s];
+    op.kind = pv::kOpStage, op.cur = 0, op.coef[0] = 0.5;
+    op.ku = b->kx[0], op.kv = b->ky[0], op.xn = b->xa, op.yn = b->ya;
+    launch(b, x0, y0, op, false, pv::kPassAll);
+    op.cur = 1, op.coef[0] = 0.0, op.coef[1] = 0.5;
+    op.ku = b->kx[1], op.kv = b->ky[1], op.xn = b->xb, op.yn = b->yb;
+    launch(b, b->xa, b->ya, op, false, pv::kPassAll);
+    op.cur = 2, op.coef[1] = 0.0, op.coef[2] = 1.0;
+    op.ku = b->kx[2], op.kv = b->ky[2], op.xn = b->xa, op.yn = b->ya;
+    launch(b, b->xb, b->yb, op, false, pv::kPassAll);
+    op.kind = pv::kOpRk4Final, op.cur = 3;
+    op.ku = b->kx[3], op.kv = b->ky[3], op.xn = x1, op.yn = y1;
+    launch(b, b->xa, b->ya, op, false, pv::kPassAll);
+}
+
+void upload_slot(pv_batch *b, int k) {
+    hip_check(hipMemcpyAsync(b->inst_d + k, &b->inst_h[k], sizeof(pv::Instance),
+                             hipMemcpyHostToDevice, b->stream),
+              "upload instance");
+    hip_check(hipMemcpyAsync(b->steps_d + k, b->steps_h + k, sizeof(pv::InstanceStep),
+                             hipMemcpyHostToDevice, b->stream),
+              "upload instance state");
+}
+
+void set_owner(pv_batch *b, size_t begin, size_t n, int value) {
+    std::vector<int> o(n, value);
+    hip_check(hipMemcpyAsync(b->owner + begin, o.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                             b->stream),
+              "upload owners");
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+}
+
+void require_fixed(pv_batch *b) {
+    if (b->adaptive)
+        raise(PV_ERR_LOGIC, "ensemble holds admitted runs; fixed-step calls need pv_batch_load");
+}
+
+} // namespace
+
+extern "C" {
+
+size_t pv_batch_bytes_per_vortex(void) { return kDoublesPerVortex * sizeof(double) + 8; }
+
+pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max_instances,
+                          pv_batch **out) {
+    return guard([&] {
+        if (!out)
+            raise(PV_ERR_INVALID_ARGUMENT, "null argument");
+        *out = nullptr;
+        if (precision != PV_FP64 && precision != PV_MIXED)
+            raise(PV_ERR_INVALID_ARGUMENT, "unsupported precision");
+        if (capacity == 0 || capacity > (size_t)INT_MAX / 2 || max_instances == 0 ||
+            max_instances > (size_t)INT_MAX / 2)
+            raise(PV_ERR_INVALID_ARGUMENT, "invalid ensemble capacity");
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "count HIP devices");
+        if (device < 0 || device >= count)
+            raise(PV_ERR_INVALID_ARGUMENT, "invalid HIP device ordinal");
+        DeviceGuard dg(device);
+        auto b = std::make_unique<pv_batch>();
+        b->device = device;
+        b->precision = precision;
+        b->capacity = capacity;
+        b->max_inst = max_instances;
+        hipDeviceProp_t prop{};
+        hip_check(hipGetDeviceProperties(&prop, device), "query HIP device properties");
+        b->cu_count = std::max(1, prop.multiProcessorCount);
+        hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "create HIP stream");
+        const size_t n = capacity;
+        b->x = dalloc<double>(n, "hipMalloc(x)");
+        b->y = dalloc<double>(n, "hipMalloc(y)");
+        b->g = dalloc<double>(n, "hipMalloc(circulation)");
+        b->xn = dalloc<double>(n, "hipMalloc(next x)");
+        b->yn = dalloc<double>(n, "hipMalloc(next y)");
+        b->xa = dalloc<double>(n, "hipMalloc(stage x)");
+        b->ya = dalloc<double>(n, "hipMalloc(stage y)");
+        b->xb = dalloc<double>(n, "hipMalloc(stage x)");
+        b->yb = dalloc<double>(n, "hipMalloc(stage y)");
+        for (int s = 0; s < 7; ++s) {
+            b->kx[s] = dalloc<double>(n, "hipMalloc(x stage)");
+            b->ky[s] = dalloc<double>(n, "hipMalloc(y stage)");
+        }
+        b->u = dalloc<double>(n, "hipMalloc(u)");
+        b->v = dalloc<double>(n, "hipMalloc(v)");
+        b->owner = dalloc<int>(n, "hipMalloc(owners)");
+        b->inst_d = dalloc<pv::Instance>(max_instances, "hipMalloc(instances)");
+        b->steps_d = dalloc<pv::InstanceStep>(max_instances, "hipMalloc(instance states)");
+        hip_check(hipHostMalloc((void **)&b->steps_h, max_instances * sizeof(pv::InstanceStep),
+                                hipHostMallocDefault),
+                  "hipHostMalloc(instance states)");
+        b->inst_h.assign(max_instances, pv::Instance{});
+        for (size_t k = 0; k < max_instances; ++k) {
+            std::memset(&b->steps_h[k], 0, sizeof(pv::InstanceStep));
+            b->steps_h[k].state = pv::kIdle;
+        }
+        hip_check(hipMemcpy(b->steps_d, b->steps_h, max_instances * sizeof(pv::InstanceStep),
+                            hipMemcpyHostToDevice),
+                  "upload instance states");
+        std::vector<int> minus(n, -1);
+        hip_check(hipMemcpy(b->owner, minus.data(), n * sizeof(int), hipMemcpyHostToDevice),
+                  "clear owners");
+        b->counter = dalloc<unsigned>(1, "hipMalloc(work counter)");
+        b->st = dalloc<pv::DevStatus>(1, "hipMalloc(status)");
+        hip_check(hipHostMalloc((void **)&b->st_host, sizeof(pv::DevStatus), hipHostMallocDefault),
+                  "hipHostMalloc(status)");
+        b->used.assign(max_instances, 0);
+        b->in_items.assign(max_instances, 0);
+        b->free_ranges = {{0, capacity}};
+        reset_status(b.get());
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        *out = b.release();
+    });
+}
+
+void pv_batch_destroy(pv_batch *b) {
+    if (!b)
+        return;
+    try {
+        DeviceGuard dg(b->device);
+        delete b;
+    } catch (...) {
+        delete b;
+    }
+}
+
+pv_status pv_batch_set_precision(pv_batch *b, int precision) {
+    return guard([&] {
+        checked(b);
+        if (precision != PV_FP64 && precision != PV_MIXED)
+            raise(PV_ERR_INVALID_ARGUMENT, "unsupported precision");
+        b->precision = precision;
+        for (auto &kv : b->graphs)
+            (void)hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        b->items_dirty = true;
+    });
+}
+
+pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const size_t *counts,
+                        size_t ninst, const double *x, const double *y, const double *gamma) {
+    return guard([&] {
+        checked(b);
+        if (ninst > b->max_inst)
+            raise(PV_ERR_INVALID_ARGUMENT, "more instances than the ensemble was created for");
+        size_t total = 0;
+        for (size_t k = 0; k < ninst; ++k)
+            total += counts[k];
+        if (total > b->capacity)
+            raise(PV_ERR_INVALID_ARGUMENT, "more vortices than the ensemble was created for");
+        DeviceGuard dg(b->device);
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        size_t off = 0;
+        std::vector<int> owner(total);
+        std::vector<int> all;
+        for (size_t k = 0; k < ninst; ++k) {
+            const pv_config c = to_config(configs[k], b->precision);
+            check_config(c);
+            validate_instance(configs[k], x + off, y + off, gamma + off, counts[k]);
+            b->inst_h[k].gp = geo_params(c);
+            b->inst_h[k].offset = (int)off;
+            b->inst_h[k].n = (int)counts[k];
+            std::fill(owner.begin() + off, owner.begin() + off + counts[k], (int)k);
+            if (counts[k])
+                all.push_back((int)k);
+            off += counts[k];
+        }
+        const size_t bytes = total * sizeof(double);
+        hip_check(hipMemcpyAsync(b->x, x, bytes, hipMemcpyHostToDevice, b->stream), "copy x");
+        hip_check(hipMemcpyAsync(b->y, y, bytes, hipMemcpyHostToDevice, b->stream), "copy y");
+        hip_check(hipMemcpyAsync(b->g, gamma, bytes, hipMemcpyHostToDevice, b->stream),
+                  "copy circulation");
+        hip_check(hipMemcpyAsync(b->owner, owner.data(), total * sizeof(int),
+                                 hipMemcpyHostToDevice, b->stream),
+                  "copy owners");
+        hip_check(hipMemcpyAsync(b->inst_d, b->inst_h.data(), ninst * sizeof(pv::Instance),
+                                 hipMemcpyHostToDevice, b->stream),
+                  "copy instances");
+        build_items(b, all);
+        b->total = total;
+        b->nfixed = ninst;
+        b->adaptive = false;
+        for (auto &kv : b->graphs)
+            (void)hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        reset_status(b);
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    });
+}
+
+pv_status pv_batch_rk4(pv_batch *b, double dt, int steps, int use_graph) {
+    return guard([&] {
+        checked(b);
+        require_fixed(b);
+        if (!std::isfinite(dt) || !(dt > 0.0))
+            raise(PV_ERR_INVALID_ARGUMENT, "timestep must be finite and positive");
+        if (steps < 0)
+            raise(PV_ERR_INVALID_ARGUMENT, "step count must be non-negative");
+        if (steps == 0 || b->nitems == 0)
+            return;
+        DeviceGuard dg(b->device);
+        reset_status(b);
+        int done = 0;
+        if (use_graph && steps >= 2) {
+            // Two steps per replay: x -> xn -> x, so the captured pointers stay valid.
+            const auto key = std::make_pair(dt, false);
+            auto it = b->graphs.find(key);
+            if (it == b->graphs.end()) {
+                hipGraph_t graph = nullptr;
+                hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
+                          "begin HIP graph capture");
+                try {
+                    enqueue_rk4(b, dt, b->x, b->y, b->xn, b->yn);
+                    enqueue_rk4(b, dt, b->xn, b->yn, b->x, b->y);
+                } catch (...) {
+                    (void)hipStreamEndCapture(b->stream, &graph);
+                    if (graph)
+                        (void)hipGraphDestroy(graph);
+                    throw;
+                }
+                hip_check(hipStreamEndCapture(b->stream, &graph), "end HIP graph capture");
+                hipGraphExec_t exec = nullptr;
+                const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                hip_check(e, "instantiate HIP graph");
+                it = b->graphs.emplace(key, exec).first;
+            }
+            for (; done + 2 <= steps; done += 2)
+                hip_check(hipGraphLaunch(it->second, b->stream), "launch HIP graph");
+        }
+        for (; done < steps; ++done) {
+            enqueue_rk4(b, dt, b->x, b->y, b->xn, b->yn);
+            std::swap(b->x, b->xn);
+            std::swap(b->y, b->yn);
+        }
+        finish_fixed(b);
+    });
+}
+
+pv_status pv_batch_read(pv_batch *b, double *x, double *y, size_t total) {
+    return guard([&] {
+        checked(b);
+        require_fixed(b);
+        if (total != b->total)
+            raise(PV_ERR_RUNTIME, "host and HIP vortex populations differ");
+        DeviceGuard dg(b->device);
+        hip_check(hipMemcpyAsync(x, b->x, total * sizeof(double), hipMemcpyDeviceToHost,
+                                 b->stream),
+                  "download x");
+        hip_check(hipMemcpyAsync(y, b->y, total * sizeof(double), hipMemcpyDeviceToHost,
+                                 b->stream),
+                  "download y");
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    });
+}
+
+pv_status pv_batch_evaluate(pv_batch *b, double *u, double *v, size_t total) {
+    return guard([&] {
+        checked(b);
+        require_fixed(b);
+        if (total != b->total)
+            raise(PV_ERR_RUNTIME, "host and HIP vortex populations differ");
+        if (total == 0)
+            return;
+        DeviceGuard dg(b->device);
+        reset_status(b);
+        pv::StageOp op{};
+        op.kind = pv::kOpStore;
+        op.ku = b->u, op.kv = b->v;
+        launch(b, b->x, b->y, op, false, pv::kPassAll);
+        hip_check(hipMemcpyAsync(u, b->u, total * sizeof(double), hipMemcpyDeviceToHost,
+                                 b->stream),
+                  "download u");
+        hip_check(hipMemcpyAsync(v, b->v, total * sizeof(double), hipMemcpyDeviceToHost,
+                                 b->stream),
+                  "download v");
+        finish_fixed(b);
+        for (size_t i = 0; i < total; ++i)
+            if (!std::isfinite(u[i]) || !std::isfinite(v[i]))
+                raise(PV_ERR_RUNTIME, "non-finite velocity; check scales and close encounters");
+    });
+}
+
+pv_status pv_batch_admit(pv_batch *b, const pv_instance_config *config, const double *x,
+                         const double *y, const double *gamma, size_t n, double t_end,
+                         double dt0, double atol, double rtol, double min_dt, double max_dt,
+                         int *slot) {
+    return guard([&] {
+        checked(b);
+        if (!config || !slot)
+            raise(PV_ERR_INVALID_ARGUMENT, "null argument");
+        *slot = -1;
+        const pv_config c = to_config(*config, b->precision);
+        check_config(c);
+        if (n == 0)
+            raise(PV_ERR_INVALID_ARGUMENT, "an admitted run needs at least one vortex");
+        validate_instance(*config, x, y, gamma, n);
+        if (!std::isfinite(dt0) || !(dt0 > 0.0) || !std::isfinite(t_end) || t_end < 0.0)
+            raise(PV_ERR_INVALID_ARGUMENT, "timestep must be finite and positive");
+        if (!b->adaptive) {
+            // First admission turns a loaded fixed-step ensemble into an adaptive one.
+            b->adaptive = true;
+            b->total = 0;
+            b->nitems = 0;
+        }
+        int k = -1;
+        for (size_t s = 0; s < b->max_inst; ++s)
+            if (!b->used[s]) {
+                k = (int)s;
+                break;
+            }
+        if (k < 0)
+            return;
+        size_t r = b->free_ranges.size();
+        for (size_t q = 0; q < b->free_ranges.size(); ++q)
+            if (b->free_ranges[q].second - b->free_ranges[q].first >= n) {
+                r = q;
+                break;
+            }
+        if (r == b->free_ranges.size())
+            return;
+        const size_t off = b->free_ranges[r].first;
+        b->free_ranges[r].first += n;
+        if (b->free_ranges[r].first == b->free_ranges[r].second)
+            b->free_ranges.erase(b->free_ranges.begin() + r);
+        DeviceGuard dg(b->device);
+        const size_t bytes = n * sizeof(double);
+        hip_check(hipMemcpyAsync(b->x + off, x, bytes, hipMemcpyHostToDevice, b->stream), "copy x");
+        hip_check(hipMemcpyAsync(b->y + off, y, bytes, hipMemcpyHostToDevice, b->stream), "copy y");
+        hip_check(hipMemcpyAsync(b->g + off, gamma, bytes, hipMemcpyHostToDevice, b->stream),
+                  "copy circulation");
+        b->inst_h[k].gp = geo_params(c);
+        b->inst_h[k].offset = (int)off;
+        b->inst_h[k].n = (int)n;
+        pv::InstanceStep &s = b->steps_h[k];
+        std::memset(&s, 0, sizeof s);
+        // src/main.cpp:162-165 (initial dt) and 312-314 (landing on the end time).
+        const double dt = std::clamp(dt0, min_dt, max_dt);
+        s.t = 0.0;
+        s.t_end = t_end;
+        s.sugg = dt;
+        s.dt = std::min(dt, t_end);
+        s.atol = atol, s.rtol = rtol, s.min_dt = min_dt, s.max_dt = max_dt;
+        s.state = t_end > 0.0 ? pv::kRunning : pv::kFinished;
+        s.fsal = 0;
+        upload_slot(b, k);
+        set_owner(b, off, n, k);
+        b->used[k] = 1;
+        b->items_dirty = true;
+        *slot = k;
+    });
+}
+
+pv_status pv_batch_attempt(pv_batch *b) {
+    return guard([&] {
+        checked(b);
+        if (!b->adaptive)
+            raise(PV_ERR_LOGIC, "no admitted runs in this ensemble");
+        DeviceGuard dg(b->device);
+        if (b->items_dirty) {
+            std::vector<int> running;
+            b->in_items.assign(b->max_inst, 0);
+            for (size_t k = 0; k < b->max_inst; ++k)
+                if (b->used[k] && b->steps_h[k].state == pv::kRunning) {
+                    running.push_back((int)k);
+                    b->in_items[k] = 1;
+                }
+            build_items(b, running);
+            b->items_dirty = false;
+        }
+        if (b->nitems == 0)
+            return;
+        // Per-vortex passes only need to reach the end of the highest occupied range.
+        size_t high = 0;
+        for (size_t k = 0; k < b->max_inst; ++k)
+            if (b->used[k])
+                high = std::max(high, (size_t)b->inst_h[k].offset + (size_t)b->inst_h[k].n);
+        const int n = (int)high;
+        pv::StageOp op{};
+        for (int s = 0; s < 7; ++s) {
+            op.kx[s] = b->kx[s], op.ky[s] = b->ky[s];
+            op.w[s] = kB5[s] - kB4[s];
+        }
+        op.x0 = b->x, op.y0 = b->y;
+        // FSAL derivative for runs that have none yet (src/timestep.cpp:111-112).
+        op.kind = pv::kOpStore;
+        op.ku = b->kx[0], op.kv = b->ky[0];
+        launch(b, b->x, b->y, op, true, pv::kPassNeedK0);
+        // Stage positions: 1 -> A, 2 -> B, 3 -> A, 4 -> B, 5 -> A, 6 (fifth order) -> next.
+        double *px[7] = {nullptr, b->xa, b->xb, b->xa, b->xb, b->xa, b->xn};
+        double *py[7] = {nullptr, b->ya, b->yb, b->ya, b->yb, b->ya, b->yn};
+        op.kind = pv::kOpStage, op.cur = 0, op.coef[0] = kDopriA[1][0];
+        op.xn = px[1], op.yn = py[1];
+        pv::launch_batch_stage1(op, b->owner, b->steps_d, n, b->stream);
+        hip_check(hipGetLastError(), "launch HIP ensemble stage");
+        for (int s = 1; s < 7; ++s) {
+            op.cur = s;
+            op.ku = b->kx[s], op.kv = b->ky[s];
+            if (s < 6) {
+                op.kind = pv::kOpStage;
+                for (int j = 0; j <= s; ++j)
+                    op.coef[j] = kDopriA[s + 1][j];
+                op.xn = px[s + 1], op.yn = py[s + 1];
+            } else {
+                op.kind = pv::kOpDopriError;
+                op.xn = nullptr, op.yn = nullptr;
+            }
+            launch(b, px[s], py[s], op, true, pv::kPassAll);
+        }
+        pv::launch_batch_control(b->steps_d, (int)b->max_inst, b->stream);
+        hip_check(hipGetLastError(), "launch HIP ensemble controller");
+        pv::launch_batch_accept(b->owner, b->steps_d, b->x, b->y, b->xn, b->yn, b->kx[0],
+                                b->ky[0], b->kx[6], b->ky[6], n, b->stream);
+        hip_check(hipGetLastError(), "launch HIP ensemble accept");
+        hip_check(hipMemcpyAsync(b->steps_h, b->steps_d, b->max_inst * sizeof(pv::InstanceStep),
+                                 hipMemcpyDeviceToHost, b->stream),
+                  "read instance states");
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        for (size_t k = 0; k < b->max_inst; ++k)
+            if (b->in_items[k] && b->steps_h[k].state != pv::kRunning)
+                b->items_dirty = true;
+    });
+}
+
+pv_status pv_batch_poll(pv_batch *b, pv_slot_state *states, size_t cap, size_t *count) {
+    return guard([&] {
+        checked(b);
+        if (!count)
+            raise(PV_ERR_INVALID_ARGUMENT, "null count");
+        size_t m = 0;
+        for (size_t k = 0; k < b->max_inst; ++k) {
+            if (!b->used[k])
+                continue;
+            if (m < cap && states) {
+                const pv::InstanceStep &s = b->steps_h[k];
+                pv_slot_state &o = states[m];
+                o.slot = (int)k;
+                o.state = s.state;
+                o.t = s.t;
+                o.dt = s.dt;
+                o.last_error = s.last_error;
+                o.accepted = s.accepted;
+                o.rejected = s.rejected_total;
+                o.failure = s.failure;
+                o.n = (size_t)b->inst_h[k].n;
+            }
+            ++m;
+        }
+        *count = m;
+    });
+}
+
+pv_status pv_batch_fetch(pv_batch *b, int slot, double *x, double *y, size_t n) {
+    return guard([&] {
+        checked(b);
+        if (slot < 0 || (size_t)slot >= b->max_inst || !b->used[slot])
+            raise(PV_ERR_INVALID_ARGUMENT, "no run in this slot");
+        if (n != (size_t)b->inst_h[slot].n)
+            raise(PV_ERR_RUNTIME, "host and HIP vortex populations differ");
+        DeviceGuard dg(b->device);
+        const size_t off = (size_t)b->inst_h[slot].offset;
+        hip_check(hipMemcpyAsync(x, b->x + off, n * sizeof(double), hipMemcpyDeviceToHost,
+                                 b->stream),
+                  "download x");
+        hip_check(hipMemcpyAsync(y, b->y + off, n * sizeof(double), hipMemcpyDeviceToHost,
+                                 b->stream),
+                  "download y");
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    });
+}
+
+pv_status pv_batch_release(pv_batch *b, int slot) {
+    return guard([&] {
+        checked(b);
+        if (slot < 0 || (size_t)slot >= b->max_inst || !b->used[slot])
+            raise(PV_ERR_INVALID_ARGUMENT, "no run in this slot");
+        DeviceGuard dg(b->device);
+        const size_t off = (size_t)b->inst_h[slot].offset, n = (size_t)b->inst_h[slot].n;
+        std::memset(&b->steps_h[slot], 0, sizeof(pv::InstanceStep));
+        b->steps_h[slot].state = pv::kIdle;
+        upload_slot(b, slot);
+        set_owner(b, off, n, -1);
+        b->used[slot] = 0;
+        // Return [off, off + n) to the free list, merging neighbours.
+        auto &fr = b->free_ranges;
+        auto pos = std::lower_bound(fr.begin(), fr.end(), std::make_pair(off, off + n));
+        pos = fr.insert(pos, {off, off + n});
+        if (pos + 1 != fr.end() && pos->second == (pos + 1)->first) {
+            pos->second = (pos + 1)->second;
+            fr.erase(pos + 1);
+        }
+        if (pos != fr.begin() && (pos - 1)->second == pos->first) {
+            (pos - 1)->second = pos->second;
+            fr.erase(pos);
+        }
+        b->items_dirty = true;
+    });
+}
+
+const char *pv_batch_failure_message(unsigned failure) {
+    if (failure & pv::kFailStateNonFinite)
+        return "vortex data must be finite";
+    if (failure & pv::kFailOutsideDisk)
+        return "vortex lies on or outside the disk";
+    if (failure & pv::kFailVelocity)
+        return "non-finite velocity; check scales and close encounters";
+    if (failure & pv::kFailErrorEstimate)
+        return "non-finite adaptive error estimate or invalid tolerance";
+    if (failure & pv::kFailRejected)
+        return "adaptive integrator could not satisfy tolerance";
+    return "";
+}
+
+} // extern "C"

@@ -1,0 +1,138 @@
+// pv_host.hpp -- host-side helpers shared by the C-ABI translation units.
+#pragma once
+#include "pv_hip.h"
+#include "pv_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+namespace pv {
+namespace host {
+
+inline thread_local std::string g_error;
+
+struct ApiError {
+    pv_status code;
+    std::string message;
+};
+
+[[noreturn]] inline void raise(pv_status code, const std::string &message) {
+    throw ApiError{code, message};
+}
+
+inline void hip_check(hipError_t status, const char *operation) {
+    if (status != hipSuccess)
+        raise(PV_ERR_RUNTIME, std::string(operation) + ": " + hipGetErrorString(status));
+}
+
+template <typename F> pv_status guard(F &&body) {
+    try {
+        body();
+        return PV_OK;
+    } catch (const ApiError &e) {
+        g_error = e.message;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_error = "out of host memory";
+        return PV_ERR_RUNTIME;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return PV_ERR_RUNTIME;
+    }
+}
+
+struct DeviceGuard {
+    int previous = 0;
+    bool changed = false;
+    explicit DeviceGuard(int device) {
+        hip_check(hipGetDevice(&previous), "query HIP device");
+        if (previous != device) {
+            hip_check(hipSetDevice(device), "select HIP device");
+            changed = true;
+        }
+    }
+    ~DeviceGuard() {
+        if (changed)
+            (void)hipSetDevice(previous);
+    }
+};
+
+template <typename T> void dfree(T *&p) {
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T> T *dalloc(size_t count, const char *what) {
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), what);
+    return static_cast<T *>(p);
+}
+
+
+inline pv::GeoParams geo_params(const pv_config &c) {
+    pv::GeoParams p{};
+    p.geometry = c.geometry;
+    if (c.geometry == PV_INFINITE) {
+        p.eps2 = c.core_radius * c.core_radius;
+    } else if (c.geometry == PV_PERIODIC) {
+        p.layers = c.image_layers;
+        p.L = c.box_length_x;
+        p.invL = 1.0 / c.box_length_x;
+        p.wave = 2.0 * pv::kPi / c.box_length_x;
+        p.scale = 0.5 / c.box_length_x;
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int n = 1; n <= pv::kMaxTerms; ++n) {
+            long double s = 0.0L;
+            for (int m = c.image_layers; m >= 1; --m)
+                s += expl(-2.0L * pi * n * m);
+            p.G4[n] = (double)(4.0L * s);
+            p.H4[n] = (double)(4.0L * s / n);
+            p.G4f[n] = (float)(4.0L * s);
+        }
+        long double cl = 0.0L;
+        for (int m = c.image_layers; m >= 1; --m)
+            cl += log1pl(expl(-4.0L * pi * m));
+        p.CL = (double)(2.0L * cl);
+    } else {
+        p.R = c.disk_radius;
+        p.invR = 1.0 / c.disk_radius;
+        p.R2 = c.disk_radius * c.disk_radius;
+    }
+    return p;
+}
+
+// Constructor checks of src/compute.cpp:50-56, 165-173, 258-261 and the geometry switch of
+// src/backend_common.cpp:4-13.
+inline void check_config(const pv_config &c) {
+    if (c.geometry == PV_INFINITE) {
+        const double r = c.core_radius, r2 = r * r;
+        if (!(r >= 0.0) || !std::isfinite(r2) || (r > 0.0 && r2 == 0.0))
+            raise(PV_ERR_INVALID_ARGUMENT,
+                  "core radius is negative or outside the supported numeric range");
+    } else if (c.geometry == PV_PERIODIC) {
+        const double lx = c.box_length_x, ly = c.box_length_y;
+        if (!(lx > 0.0) || !(ly > 0.0) || !std::isfinite(lx) || !std::isfinite(ly) ||
+            !std::isfinite(2.0 * pv::kPi / lx) || c.image_layers < 0 || c.image_layers > 64)
+            raise(PV_ERR_INVALID_ARGUMENT, "invalid periodic-box parameters");
+        if (std::fabs(lx - ly) > 1e-13 * std::max(lx, ly))
+            raise(PV_ERR_INVALID_ARGUMENT,
+                  "Weiss-McWilliams kernel requires a square periodic box");
+    } else if (c.geometry == PV_DISK) {
+        const double r = c.disk_radius, r2 = r * r;
+        if (!(r > 0.0) || !std::isfinite(r2) || r2 == 0.0)
+            raise(PV_ERR_INVALID_ARGUMENT, "invalid disk parameters");
+    } else {
+        raise(PV_ERR_INVALID_ARGUMENT, "unsupported boundary condition");
+    }
+    if (c.precision != PV_FP64 && c.precision != PV_MIXED)
+        raise(PV_ERR_INVALID_ARGUMENT, "unsupported precision");
+}
+
+
+} // namespace host
+} // namespace pv

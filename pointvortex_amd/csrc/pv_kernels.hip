@@ -10,92 +10,14 @@
 //             them in split order, so every target's sum has a fixed order (deterministic).
 // The stage arithmetic of RK4 / DOPRI5 is fused into the epilogue of the evaluation that
 // produces the stage (StageOp), so an RK4 step is four launches and a DOPRI5 attempt seven.
-#include "pv_kernels.hpp"
-
-#include <climits>
+#include "pv_epilogue.hpp"
 
 namespace pv {
 
-__device__ __forceinline__ bool finite64(double v) { return __builtin_isfinite(v); }
-
-__device__ __forceinline__ void record_failure(DevStatus *st, unsigned epoch, unsigned kind) {
-    atomicOr(&st->flags, kind);
-    atomicMin(&st->fail_epoch, epoch);
-}
-
-template <typename T> __device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        v = fmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-// Max over the block; result valid in every thread.
-__device__ double block_max(double v) {
-    __shared__ double red[kBlock / 64];
-    v = wave_max(v);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0)
-        red[wave] = v;
-    __syncthreads();
-    double m = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
-        m = fmax(m, red[w]);
-    return m;
-}
-
-// Per-target epilogue; returns this target's DOPRI5 error contribution (0 otherwise).
-__device__ double epilogue(const PairLaunch &a, const GeoParams &gp, int i, double u,
-                           double v) {
-    const StageOp &op = a.op;
-    if (!finite64(u) || !finite64(v))
-        record_failure(a.st, a.epoch, kFailVelocity);
-    if (op.ku) {
-        op.ku[i] = u;
-        op.kv[i] = v;
-    }
-    if (op.kind == kOpStage) {
-        // src/timestep.cpp:19-30 (makeStage), coefficients in stage order.
-        double dx = 0.0, dy = 0.0;
-        for (int j = 0; j < op.cur; ++j) {
-            dx += op.coef[j] * op.kx[j][i];
-            dy += op.coef[j] * op.ky[j][i];
-        }
-        dx += op.coef[op.cur] * u;
-        dy += op.coef[op.cur] * v;
-        op.xn[i] = op.x0[i] + op.dt * dx;
-        op.yn[i] = op.y0[i] + op.dt * dy;
-    } else if (op.kind == kOpRk4Final) {
-        // src/timestep.cpp:48-58: combination, then the finiteness check of the new state.
-        const double xn = op.x0[i] + op.dt * (op.kx[0][i] + 2.0 * op.kx[1][i] +
-                                              2.0 * op.kx[2][i] + u) / 6.0;
-        const double yn = op.y0[i] + op.dt * (op.ky[0][i] + 2.0 * op.ky[1][i] +
-                                              2.0 * op.ky[2][i] + v) / 6.0;
-        op.xn[i] = xn;
-        op.yn[i] = yn;
-        if (!finite64(xn) || !finite64(yn))
-            record_failure(a.st, a.epoch, kFailStateNonFinite);
-    } else if (op.kind == kOpDopriError) {
-        // src/timestep.cpp:83-98: max-norm of the embedded error, scaled per component.
-        double ex = 0.0, ey = 0.0;
-        for (int s = 0; s < 6; ++s) {
-            ex += op.w[s] * op.kx[s][i];
-            ey += op.w[s] * op.ky[s][i];
-        }
-        ex += op.w[6] * u;
-        ey += op.w[6] * v;
-        const double x5 = a.x[i], y5 = a.y[i];
-        const double scx = op.atol + op.rtol * fmax(fabs(op.x0[i]), fabs(x5));
-        const double scy = op.atol + op.rtol * fmax(fabs(op.y0[i]), fabs(y5));
-        if (!finite64(ex) || !finite64(ey) || !finite64(scx) || !finite64(scy) || !(scx > 0.0) ||
-            !(scy > 0.0)) {
-            record_failure(a.st, a.epoch, kFailErrorEstimate);
-            return __builtin_inf();
-        }
-        return fmax(fabs(ex) / scx, fabs(ey) / scy);
-    }
-    return 0.0;
+__device__ __forceinline__ double epilogue(const PairLaunch &a, const GeoParams &, int i,
+                                           double u, double v) {
+    return stage_epilogue(a.op, a.op.dt, a.op.atol, a.op.rtol, FailSink{a.st, a.epoch, nullptr},
+                          a.x, a.y, i, u, v);
 }
 
 template <class P, int TPT, int TILE>

@@ -31,7 +31,7 @@ struct StageOp {
     int cur;                   // stage index of this evaluation
     double dt;
     double coef[7];            // kOpStage: next position = x0 + dt * sum_{j<=cur} coef[j] k_j
-    double w[7];               // kOpDopriError: dt (b5 - b4)
+    double w[7];               // kOpDopriError: b5 - b4 (the epilogue scales by dt)
     double atol, rtol;
     double *ku, *kv;           // where this evaluation's velocity is stored (may be null)
     const double *kx[7], *ky[7];  // earlier stage velocities

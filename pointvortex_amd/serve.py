@@ -1,0 +1,173 @@
+"""A request loop with continuous batching over adaptive ensembles.
+
+Requests (geometry, N, circulations, horizon, tolerance) arrive over time.  Between DOPRI5
+attempts the loop admits every arrived request whose vortices fit (the HBM budget is the
+ensemble capacity, 208 bytes per vortex), so new runs join runs already in flight; finished or
+failed runs leave and free their slots.  Each run keeps its state, FSAL derivative and
+step-size controller on the device; one attempt advances every running run by one DOPRI5
+attempt in seven persistent launches.  Latency is measured from arrival to completion.
+
+Precision policy: a request runs in mixed precision when its rtol is at least
+`mixed_rtol_threshold` (1e-6 by default), otherwise in FP64; each precision has its own
+ensemble.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import workloads as wl
+from .ensemble import Ensemble, InstanceConfig, bytes_per_vortex
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float            # seconds after the loop starts
+    geometry: str
+    n: int
+    seed: int
+    t_end: float
+    rtol: float
+    atol: float
+    dt0: float = 1e-3
+
+    def population(self):
+        if self.geometry == "infinite":
+            return wl.infinite(self.n, self.seed), InstanceConfig("infinite")
+        if self.geometry == "periodic":
+            return (wl.periodic(self.n, self.seed, box_length=2 * np.pi),
+                    InstanceConfig("periodic", box_length=2 * np.pi))
+        return wl.disk(self.n, self.seed), InstanceConfig("disk", disk_radius=1.0)
+
+
+@dataclass
+class Result:
+    rid: int
+    state: str
+    arrival: float
+    admitted: float
+    finished: float
+    accepted: int
+    rejected: int
+    failure: str
+    precision: str
+    x: Optional[np.ndarray] = None
+    y: Optional[np.ndarray] = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+@dataclass
+class ServerStats:
+    results: List[Result] = field(default_factory=list)
+    wall: float = 0.0
+    attempts: int = 0
+
+    def summary(self):
+        lat = np.array([r.latency for r in self.results if r.state == "finished"])
+        done = len(lat)
+        return {
+            "completed": done,
+            "failed": sum(r.state == "failed" for r in self.results),
+            "wall_s": self.wall,
+            "solves_per_s": done / self.wall if self.wall > 0 else 0.0,
+            "p50_s": float(np.percentile(lat, 50)) if done else None,
+            "p99_s": float(np.percentile(lat, 99)) if done else None,
+            "attempts": self.attempts,
+        }
+
+
+class Server:
+    def __init__(self, hbm_budget_bytes: int, max_runs: int = 4096, device: int = 0,
+                 mixed_rtol_threshold: float = 1e-6, keep_results: bool = False):
+        per = bytes_per_vortex()
+        self.capacity = max(1, int(hbm_budget_bytes // per))
+        self.max_runs = max_runs
+        self.device = device
+        self.threshold = mixed_rtol_threshold
+        self.keep = keep_results
+        self.engines: Dict[str, Ensemble] = {}
+
+    def precision_for(self, req: Request) -> str:
+        return "mixed" if req.rtol >= self.threshold else "fp64"
+
+    def engine(self, precision: str) -> Ensemble:
+        if precision not in self.engines:
+            self.engines[precision] = Ensemble(self.capacity, self.max_runs, precision,
+                                               self.device)
+        return self.engines[precision]
+
+    def run(self, requests: List[Request], clock=time.perf_counter,
+            max_wall: float = float("inf")) -> ServerStats:
+        pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        active: Dict[tuple, tuple] = {}
+        stats = ServerStats()
+        t0 = clock()
+        while pending or active:
+            now = clock() - t0
+            if now > max_wall:
+                break
+            # Admit, in arrival order, everything that has arrived and fits.
+            while pending and pending[0].arrival <= now:
+                req = pending[0]
+                prec = self.precision_for(req)
+                pop, cfg = req.population()
+                slot = self.engine(prec).admit(cfg, pop.x, pop.y, pop.circulation, req.t_end,
+                                               req.dt0, req.atol, req.rtol)
+                if slot is None:
+                    if not any(k[0] == prec for k in active):
+                        raise RuntimeError(f"request {req.rid} (N={req.n}) exceeds the budget")
+                    break
+                active[(prec, slot)] = (req, clock() - t0)
+                pending.popleft()
+            if not active:
+                wait = pending[0].arrival - (clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+                continue
+            for prec, eng in self.engines.items():
+                if any(k[0] == prec for k in active):
+                    eng.attempt()
+                    stats.attempts += 1
+            now = clock() - t0
+            for prec, eng in self.engines.items():
+                for s in eng.poll():
+                    key = (prec, s.slot)
+                    if key not in active or s.state == "running":
+                        continue
+                    req, admitted = active.pop(key)
+                    x = y = None
+                    if self.keep and s.state == "finished":
+                        x, y = eng.fetch(s.slot, s.n)
+                    eng.release(s.slot)
+                    stats.results.append(Result(req.rid, s.state, req.arrival, admitted, now,
+                                                s.accepted, s.rejected, s.failure, prec, x, y))
+        stats.wall = clock() - t0
+        return stats
+
+
+def make_requests(count: int, seed: int, rate: float, n_min: int = 64, n_max: int = 131072,
+                  t_min: float = 0.1, t_max: float = 1.0, rtol_min: float = 1e-10,
+                  rtol_max: float = 1e-6, atol_factor: float = 1e-3) -> List[Request]:
+    """Open-loop Poisson arrivals at `rate` per second; geometry uniform over the three;
+    N log-uniform in [n_min, n_max] (even); rtol log-uniform; horizon uniform; atol =
+    atol_factor * rtol; each request seeded by its index."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        geo = ["infinite", "periodic", "disk"][int(rng.integers(0, 3))]
+        n = int(np.floor(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        n = max(2, n - n % 2)
+        rtol = float(np.exp(rng.uniform(np.log(rtol_min), np.log(rtol_max))))
+        horizon = float(rng.uniform(t_min, t_max))
+        out.append(Request(k, t, geo, n, seed * 1_000_003 + k, horizon, rtol, atol_factor * rtol))
+    return out

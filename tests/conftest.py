@@ -43,3 +43,35 @@ def per_vortex(u, v, ur, vr, floor_fraction=1e-3):
     floor = floor_fraction * np.sqrt(np.mean(mag ** 2)) if len(mag) else 0.0
     err = np.sqrt((u - ur) ** 2 + (v - vr) ** 2)
     return float(np.max(err / np.maximum(mag, floor))) if len(mag) else 0.0
+
+
+def disk_longdouble(x, y, g, radius=1.0):
+    """Disk velocities (src/compute.cpp:301-333 formula) in x87 extended precision, as a
+    truth reference for cases where the FP64 reference itself loses digits near the wall."""
+    L = np.longdouble
+    x, y, g = (np.asarray(a, dtype=L) for a in (x, y, g))
+    R = L(radius)
+    inv2pi = L(0.5) / L(np.pi)
+    tx, ty = x[:, None] / R, y[:, None] / R
+    sx, sy = x[None, :] / R, y[None, :] / R
+    dx, dy = x[:, None] - x[None, :], y[:, None] - y[None, :]
+    r2 = dx * dx + dy * dy
+    np.fill_diagonal(r2, L(1))
+    c = inv2pi * g[None, :] / r2
+    np.fill_diagonal(c, L(0))
+    a = L(1) - (tx * sx + ty * sy)
+    b = ty * sx - tx * sy
+    k = -inv2pi * g[None, :] / R / (a * a + b * b)
+    u = (-c * dy).sum(1) - (k * (-a * sy + b * sx)).sum(1)
+    v = (c * dx).sum(1) + (k * (-a * sx - b * sy)).sum(1)
+    return u.astype(np.float64), v.astype(np.float64)
+
+
+def fp64_gate(geometry, x, y, g, ur, vr, radius=1.0):
+    """Normwise tolerance for FP64 parity against the oracle: 1e-12, or twice the oracle's own
+    distance from an extended-precision sum when that is larger (a vortex within ~1e-4 of the
+    disk wall makes 1 - t.s cancel in the reference's FP64 arithmetic)."""
+    if geometry != "disk" or len(x) > 3000:
+        return 1e-12
+    ue, ve = disk_longdouble(x, y, g, radius)
+    return max(1e-12, 2.0 * normwise(ur, vr, ue, ve))

@@ -1,0 +1,235 @@
+"""Ensembles: every instance against the CPU oracle, hipGraph replay against eager launches,
+and adaptive runs alone against the same runs in a mix (they must agree bit for bit: each
+target's sum is computed by one wave in a fixed order whatever else is scheduled)."""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import spec
+from pointvortex_amd import HipKernel, SimParams, VortexSystem
+from pointvortex_amd import workloads as wl
+from pointvortex_amd.ensemble import Ensemble, InstanceConfig, bytes_per_vortex
+from tests.conftest import fp64_gate, normwise
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_population(count, seed, n_min=2, n_max=300):
+    rng = np.random.default_rng(seed)
+    cfgs, pops = [], []
+    for k in range(count):
+        geo = ["infinite", "periodic", "disk"][k % 3]
+        n = int(rng.integers(n_min, n_max + 1))
+        n += n % 2
+        if geo == "infinite":
+            p = wl.infinite(n, 100 + k)
+            cfgs.append(InstanceConfig("infinite", core_radius=0.01 if k % 2 else 0.0))
+        elif geo == "periodic":
+            p = wl.periodic(n, 100 + k, box_length=1.0)
+            cfgs.append(InstanceConfig("periodic", box_length=1.0, image_layers=8))
+        else:
+            p = wl.disk(n, 100 + k, wall_fraction=0.0)
+            cfgs.append(InstanceConfig("disk", disk_radius=1.0))
+        pops.append(p)
+    return cfgs, pops
+
+
+def ospec(c):
+    return spec(c.geometry, core_radius=c.core_radius, box_length=c.box_length,
+                image_layers=c.image_layers, radius=c.disk_radius)
+
+
+def oracle_rk4(oracle, c, x, y, g, dt, steps):
+    x, y = x.copy(), y.copy()
+    it = oracle.integrator(len(x))
+    for _ in range(steps):
+        it.rk4_step(ospec(c), x, y, g, dt)
+    return x, y
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
+def test_ensemble_evaluate_matches_oracle(oracle, precision, tol):
+    cfgs, pops = mixed_population(30, 1)
+    e = Ensemble(sum(p.n for p in pops), 64, precision)
+    e.load(cfgs, [p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
+    us, vs = e.evaluate()
+    for c, p, u, v in zip(cfgs, pops, us, vs):
+        ur, vr = oracle.evaluate(ospec(c), p.x, p.y, p.circulation)
+        gate = fp64_gate(c.geometry, p.x, p.y, p.circulation, ur, vr) if tol < 1e-6 else tol
+        assert normwise(u, v, ur, vr) <= gate, (c, p.n)
+
+
+def test_ensemble_rk4_graph_matches_eager_and_oracle(oracle):
+    cfgs, pops = mixed_population(24, 2)
+    args = ([p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
+    total = sum(p.n for p in pops)
+    a = Ensemble(total, 32)
+    b = Ensemble(total, 32)
+    a.load(cfgs, *args)
+    b.load(cfgs, *args)
+    a.rk4(1e-3, 6, use_graph=True)
+    b.rk4(1e-3, 6, use_graph=False)
+    xa, ya = a.read()
+    xb, yb = b.read()
+    for k in range(len(pops)):
+        assert np.array_equal(xa[k], xb[k]) and np.array_equal(ya[k], yb[k])
+    for k, (c, p, x, y) in enumerate(zip(cfgs, pops, xa, ya)):
+        xr, yr = oracle_rk4(oracle, c, p.x, p.y, p.circulation, 1e-3, 6)
+        # The FP64 velocity bar (normwise 1e-12, or the oracle's own distance from an
+        # extended-precision sum near a disk wall) carries over to the displacement as far as
+        # the instance's conditioning allows: a close pair spinning at omega * dt >> 1 makes
+        # RK4 amplify any velocity difference.  The oracle measures that amplification itself
+        # by rerunning with circulations perturbed by the bar (velocity is linear in them).
+        ur, vr = oracle.evaluate(ospec(c), p.x, p.y, p.circulation)
+        gate = fp64_gate(c.geometry, p.x, p.y, p.circulation, ur, vr)
+        rng = np.random.default_rng(k)
+        g = p.circulation * (1.0 + gate * rng.standard_normal(p.n))
+        if c.geometry == "periodic":
+            g -= g.mean()
+        xq, yq = oracle_rk4(oracle, c, p.x, p.y, g, 1e-3, 6)
+        amplified = normwise(xq - p.x, yq - p.y, xr - p.x, yr - p.y)
+        err = normwise(x - p.x, y - p.y, xr - p.x, yr - p.y)
+        assert err <= max(100 * gate, 10 * amplified), (k, c, p.n, err, gate, amplified)
+
+
+def test_ensemble_graph_replays_repeat(oracle):
+    """Replays of one captured graph, with eager work and synchronisation in between, give the
+    same states as eager launches."""
+    cfgs, pops = mixed_population(9, 3)
+    args = ([p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
+    total = sum(p.n for p in pops)
+    a, b = Ensemble(total, 16), Ensemble(total, 16)
+    a.load(cfgs, *args)
+    b.load(cfgs, *args)
+    for _ in range(3):
+        a.rk4(1e-3, 4, use_graph=True)
+        a.evaluate()
+        b.rk4(1e-3, 4, use_graph=False)
+        b.evaluate()
+        xa, _ = a.read()
+        xb, _ = b.read()
+        assert all(np.array_equal(p, q) for p, q in zip(xa, xb))
+
+
+def drive(ens, max_attempts):
+    """Attempts until no run is left running; a bounded loop, so a run that stops making
+    progress fails the test instead of spinning."""
+    for _ in range(max_attempts):
+        if not any(s.state == "running" for s in ens.poll()):
+            return
+        ens.attempt()
+    raise AssertionError(f"runs still running after {max_attempts} attempts: {ens.poll()}")
+
+
+def run_alone_and_mixed(precision):
+    rng = np.random.default_rng(7)
+    target = wl.periodic(64, 11, box_length=1.0)
+    cfg = InstanceConfig("periodic", box_length=1.0)
+    kw = dict(t_end=0.002, dt0=1e-4, atol=1e-10, rtol=1e-8)
+    alone = Ensemble(4096, 8, precision)
+    s0 = alone.admit(cfg, target.x, target.y, target.circulation, **kw)
+    drive(alone, 200)
+    st_alone = [s for s in alone.poll() if s.slot == s0][0]
+    xa, ya = alone.fetch(s0, target.n)
+    mix = Ensemble(20000, 16, precision)
+    others = []
+    for k in range(6):
+        p = wl.disk(int(rng.integers(50, 900)), 50 + k, wall_fraction=0.0, gaussian=True)
+        others.append(mix.admit(InstanceConfig("disk"), p.x, p.y, p.circulation, t_end=0.01,
+                                rtol=1e-7))
+    mix.attempt()
+    mix.attempt()
+    s1 = mix.admit(cfg, target.x, target.y, target.circulation, **kw)
+    drive(mix, 2000)
+    st_mix = [s for s in mix.poll() if s.slot == s1][0]
+    xm, ym = mix.fetch(s1, target.n)
+    return st_alone, st_mix, (xa, ya), (xm, ym), mix.poll()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_adaptive_run_alone_equals_run_in_the_mix(precision):
+    st_alone, st_mix, (xa, ya), (xm, ym), everyone = run_alone_and_mixed(precision)
+    assert st_alone.state == st_mix.state == "finished"
+    assert st_alone.accepted == st_mix.accepted and st_alone.rejected == st_mix.rejected
+    assert np.array_equal(xa, xm) and np.array_equal(ya, ym)
+    assert all(s.state == "finished" for s in everyone)
+
+
+def test_adaptive_matches_single_handle_dopri5():
+    """One adaptive run against the single-handle DOPRI5 path driven to the same end time with
+    the reference's landing rule (src/main.cpp:312-314)."""
+    p = wl.infinite(100, 5)
+    kw = dict(atol=1e-12, rtol=1e-9, min_dt=1e-12, max_dt=0.1)
+    e = Ensemble(1000, 4)
+    slot = e.admit(InstanceConfig("infinite"), p.x, p.y, p.circulation, t_end=0.002, dt0=1e-4,
+                   **kw)
+    drive(e, 500)
+    xe, ye = e.fetch(slot, p.n)
+    k = HipKernel(SimParams(boundaryCondition="infinite"))
+    s = VortexSystem(p.x, p.y, p.circulation)
+    k.uploadDeviceState(s)
+    t, dt, steps = 0.0, 1e-4, 0
+    for _ in range(500):
+        if t >= 0.002:
+            break
+        r = k.deviceDopri5Step(min(dt, 0.002 - t), kw["atol"], kw["rtol"], kw["min_dt"],
+                               kw["max_dt"])
+        t += r.acceptedTimeStep
+        dt = r.suggestedTimeStep
+        steps += 1
+        if t + 16 * 2.220446049250313e-16 * abs(t) >= 0.002:
+            break
+    out = s.copy()
+    k.downloadDeviceState(out)
+    st = [q for q in e.poll() if q.slot == slot][0]
+    assert st.accepted == steps
+    scale = np.max(np.abs(np.concatenate([out.x, out.y])))
+    assert np.max(np.abs(xe - out.x)) <= 1e-10 * scale
+    assert np.max(np.abs(ye - out.y)) <= 1e-10 * scale
+
+
+def test_admission_by_footprint_and_release():
+    e = Ensemble(1000, 4)
+    p = wl.infinite(600, 1)
+    q = wl.infinite(500, 2)
+    a = e.admit(InstanceConfig(), p.x, p.y, p.circulation, t_end=1e-3)
+    assert a is not None
+    assert e.admit(InstanceConfig(), q.x, q.y, q.circulation, t_end=1e-3) is None
+    e.release(a)
+    b = e.admit(InstanceConfig(), q.x, q.y, q.circulation, t_end=1e-3)
+    assert b is not None
+    assert bytes_per_vortex() >= 200
+
+
+def test_failed_run_does_not_stop_others():
+    e = Ensemble(2000, 4)
+    good = wl.infinite(100, 3)
+    a = e.admit(InstanceConfig(), good.x, good.y, good.circulation, t_end=0.005)
+    bad_x = good.x.copy()
+    bad_x[1] = bad_x[0]
+    bad_y = good.y.copy()
+    bad_y[1] = bad_y[0]
+    b = e.admit(InstanceConfig(), bad_x, bad_y, good.circulation, t_end=0.005)
+    drive(e, 2000)
+    st = {s.slot: s for s in e.poll()}
+    assert st[a].state == "finished"
+    assert st[b].state == "failed" and "non-finite velocity" in st[b].failure
+
+
+def test_serving_loop_completes_and_matches_alone():
+    """Requests arriving while others run: every result equals the same request served alone."""
+    from pointvortex_amd.serve import Server, make_requests
+
+    reqs = make_requests(12, 5, rate=400.0, n_min=32, n_max=200, t_min=0.0005, t_max=0.002,
+                         rtol_min=1e-8, rtol_max=1e-6)
+    srv = Server(hbm_budget_bytes=4_000_000, max_runs=8, keep_results=True)
+    stats = srv.run(reqs, max_wall=60.0)
+    summary = stats.summary()
+    assert summary["completed"] == len(reqs) and summary["failed"] == 0
+    assert summary["p99_s"] >= summary["p50_s"] > 0
+    by_id = {r.rid: r for r in stats.results}
+    for req in reqs[:4]:
+        alone = Server(hbm_budget_bytes=4_000_000, max_runs=2, keep_results=True)
+        req0 = type(req)(**{**req.__dict__, "arrival": 0.0})
+        r = alone.run([req0], max_wall=30.0).results[0]
+        assert r.accepted == by_id[req.rid].accepted
+        assert np.array_equal(r.x, by_id[req.rid].x) and np.array_equal(r.y, by_id[req.rid].y)

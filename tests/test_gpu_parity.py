@@ -13,7 +13,7 @@ from oracle.pyoracle import spec
 from pointvortex_amd import (BackendRuntimeError, HipKernel, InvalidArgument, LogicError,
                              OutOfRange, SimParams, VortexSystem)
 from pointvortex_amd import workloads as wl
-from tests.conftest import normwise, per_vortex
+from tests.conftest import fp64_gate, normwise, per_vortex
 
 pytestmark = pytest.mark.gpu
 
@@ -159,8 +159,9 @@ def test_fp64_velocity_parity(oracle, geometry, n):
     k = kernel_for(geometry, boxLengthX=2.0, boxLengthY=2.0)
     vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
     ur, vr = oracle.evaluate(spec_for(geometry, boxLengthX=2.0), pop.x, pop.y, pop.circulation)
-    assert normwise(vel.x, vel.y, ur, vr) <= 1e-12
-    assert per_vortex(vel.x, vel.y, ur, vr) <= 1e-10
+    gate = fp64_gate(geometry, pop.x, pop.y, pop.circulation, ur, vr)
+    assert normwise(vel.x, vel.y, ur, vr) <= gate
+    assert per_vortex(vel.x, vel.y, ur, vr) <= 100 * gate
 
 
 @pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])

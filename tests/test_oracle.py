@@ -285,3 +285,17 @@ def test_oracle_bitwise_equals_reference_build(oracle, reference, geometry):
     kc = spec(geometry, core_radius=0.03, box_length=2.0)
     assert np.array_equal(oracle.invariants(kc, pop.x, pop.y, pop.circulation),
                           reference.invariants(kc, pop.x, pop.y, pop.circulation))
+
+
+def test_longdouble_disk_helper_against_brute_force():
+    """The extended-precision disk sum used to set FP64 gates near the wall is itself pinned."""
+    from tests.conftest import disk_longdouble
+
+    rng = np.random.default_rng(8)
+    r = np.sqrt(rng.uniform(0, 1, 6)) * 0.999
+    th = rng.uniform(0, 2 * np.pi, 6)
+    x, y = r * np.cos(th), r * np.sin(th)
+    g = rng.standard_normal(6)
+    u, v = disk_longdouble(x, y, g)
+    ref = _mp_disk(x, y, g, 1.0)
+    assert normwise(u, v, ref[:, 0], ref[:, 1]) <= 1e-15
