@@ -100,6 +100,103 @@ def per_geometry(pv, n, repeats=3):
     return out
 
 
+def ensemble_configs(pops):
+    from pointvortex_amd.ensemble import InstanceConfig
+
+    out = []
+    for p in pops:
+        if p.geometry == "periodic":
+            out.append(InstanceConfig("periodic", box_length=p.box_length))
+        elif p.geometry == "disk":
+            out.append(InstanceConfig("disk", disk_radius=p.radius))
+        else:
+            out.append(InstanceConfig("infinite"))
+    return out
+
+
+def ensemble_bench(pv, steps, count=4096):
+    """The 4096-instance RK4 ensemble (geometry 50/25/25 %, N log-uniform in [32, 4096]),
+    one persistent launch per stage, `steps` steps replayed from a captured two-step hipGraph,
+    FP64 and mixed.  A solve is one instance's completed `steps`-step run; an instance whose
+    state fails (leaves the disk, non-finite velocity) stops there and is not a solve.  Pairs
+    are metered as N(N-1) per evaluation of the steps each instance completed."""
+    from pointvortex_amd import workloads as wl
+    from pointvortex_amd.ensemble import Ensemble
+
+    pops = wl.ensemble(count)
+    cfgs = ensemble_configs(pops)
+    total = sum(p.n for p in pops)
+    args = ([p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
+    out = {"instances": count, "vortices": total, "steps": steps, "dt": 1e-3,
+           "pairs_per_stage": sum(p.n * (p.n - 1) for p in pops)}
+    vel = {}
+    for prec in ("fp64", "mixed"):
+        e = Ensemble(total, count, prec)
+        e.load(cfgs, *args)
+        vel[prec] = e.evaluate()
+        e.load(cfgs, *args)
+        t0 = time.perf_counter()
+        e.rk4(1e-3, steps, use_graph=True)
+        wall = time.perf_counter() - t0
+        status = e.instance_status()
+        # Only evaluated pairs are metered: an instance that failed stopped after the steps
+        # it completed (four evaluations each; the failing step's partial work is not counted).
+        pairs = sum(4 * done * p.n * (p.n - 1) for (_, done), p in zip(status, pops))
+        solved = sum(1 for f, _ in status if not f)
+        reasons = {}
+        for (f, _), p in zip(status, pops):
+            if f:
+                key = f"{p.geometry}: {f}"
+                reasons[key] = reasons.get(key, 0) + 1
+        out[prec] = {"wall_s": wall, "solves_per_s": solved / wall, "pairs_per_s": pairs / wall,
+                     "completed_instances": solved, "failed_instances": count - solved,
+                     "failures": reasons}
+        log(f"[bench] ensemble {prec}: {wall:.2f} s, {solved / wall:.1f} solves/s, "
+            f"{pairs / wall:.3e} pairs/s, failed {count - solved}: {reasons}")
+        e.close()
+    out["mixed_speedup"] = out["mixed"]["pairs_per_s"] / out["fp64"]["pairs_per_s"]
+    err = {}
+    for geo in ("infinite", "periodic", "disk"):
+        worst = 0.0
+        for k, p in enumerate(pops):
+            if p.geometry != geo:
+                continue
+            worst = max(worst, normwise(vel["mixed"][0][k], vel["mixed"][1][k],
+                                        vel["fp64"][0][k], vel["fp64"][1][k]))
+        err[geo] = worst
+    out["mixed_max_normwise_err_vs_fp64_initial"] = err
+    return out
+
+
+def serving_bench(pv, count=128, seed=5):
+    """Continuous-batching request loop, scaled down so that it runs in about a minute:
+    N log-uniform in [64, 2048], horizon U[1e-4, 1e-3], rtol log-uniform in [1e-10, 1e-6],
+    atol = 1e-3 rtol, geometry uniform, 4 GB HBM budget.  Capacity is measured first with every
+    request arriving at t=0; the measured run then uses Poisson arrivals at 80 % of it."""
+    from pointvortex_amd.serve import Server, make_requests
+
+    def progress(phase):
+        def cb(result, stats):
+            if len(stats.results) % 16 == 0 or len(stats.results) == count:
+                log(f"[bench] serving {phase}: {len(stats.results)}/{count} done")
+        return cb
+
+    kw = dict(n_min=64, n_max=2048, t_min=1e-4, t_max=1e-3, rtol_min=1e-10, rtol_max=1e-6)
+    reqs = make_requests(count, seed, rate=0.0, **kw)
+    cap = Server(hbm_budget_bytes=4 << 30, max_runs=count).run(
+        reqs, max_wall=120.0, on_result=progress("capacity")).summary()
+    rate = 0.8 * cap["solves_per_s"]
+    reqs = make_requests(count, seed, rate=rate, **kw)
+    run = Server(hbm_budget_bytes=4 << 30, max_runs=count).run(
+        reqs, max_wall=120.0, on_result=progress("open loop")).summary()
+    log(f"[bench] serving: capacity {cap['solves_per_s']:.1f} solves/s; at {rate:.1f}/s: "
+        f"p50 {run['p50_s']:.3f} s p99 {run['p99_s']:.3f} s")
+    return {"requests": count, "capacity_solves_per_s": cap["solves_per_s"],
+            "arrival_rate_per_s": rate, **run,
+            "workload": "N log-uniform [64,2048], t U[1e-4,1e-3], rtol log-uniform "
+                        "[1e-10,1e-6], atol 1e-3 rtol, geometry uniform, seed 5"}
+
+
 def cpu_baseline(pop, budget_s=20.0):
     """Reference CPU disk kernel (src/compute.cpp:286-336) on this host, bounded sample: as many
     full N=65536 evaluations as fit in about budget_s (at least one)."""
@@ -136,8 +233,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the per-geometry table")
-    ap.add_argument("--only", choices=["fp64", "geometries", "copy"], default=None,
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the per-geometry table, the ensemble and the serving loop")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--only", choices=["fp64", "geometries", "copy", "ensemble", "serving"],
+                    default=None,
                     help="profiling runs: only the FP64 RK4 run, only the per-geometry table, "
                          "or only the 256 MiB copy-kernel calibration")
     args = ap.parse_args()
@@ -157,6 +257,12 @@ def main():
         return
     if args.only == "geometries":
         print(json.dumps(per_geometry(pv, n)), flush=True)
+        return
+    if args.only == "ensemble":
+        print(json.dumps(ensemble_bench(pv, args.ensemble_steps)), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps(serving_bench(pv)), flush=True)
         return
     pop = wl.disk(n, 2)
     pairs_per_step = 4 * n * (n - 1)
@@ -233,6 +339,8 @@ def main():
     }
     if not args.no_extra:
         result["per_geometry_n65536"] = per_geometry(pv, n)
+        result["ensemble_4096"] = ensemble_bench(pv, args.ensemble_steps)
+        result["serving"] = serving_bench(pv)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(pop)

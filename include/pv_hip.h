@@ -212,12 +212,21 @@ size_t pv_batch_bytes_per_vortex(void);
 
 /* Fixed-step ensembles: replace the contents with `ninst` instances laid out one after the
  * other in x, y, gamma (counts[k] vortices each), advance them with RK4 (optionally as
- * replays of a captured hipGraph), read positions back, or evaluate velocities. */
+ * replays of a captured hipGraph), read positions back, or evaluate velocities.
+ * Failures are per instance: where the reference's single run throws
+ * (src/compute.cpp:60-69, src/timestep.cpp:54-58; backend_cuda.cu:387-393), the failing
+ * instance stops at the input state of its failing step and the others go on.
+ * pv_batch_failures reports each instance's failure bits (0 = healthy; see
+ * pv_batch_failure_message) and, if steps_done is not null, the RK4 steps it completed
+ * since pv_batch_load; pv_batch_read returns a failed instance's stopped state and
+ * pv_batch_evaluate leaves its velocities unspecified. */
 pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const size_t *counts,
                         size_t ninst, const double *x, const double *y, const double *gamma);
 pv_status pv_batch_rk4(pv_batch *b, double dt, int steps, int use_graph);
 pv_status pv_batch_read(pv_batch *b, double *x, double *y, size_t total);
 pv_status pv_batch_evaluate(pv_batch *b, double *u, double *v, size_t total);
+pv_status pv_batch_failures(pv_batch *b, unsigned *failures, unsigned long long *steps_done,
+                            size_t ninst);
 
 /* Adaptive ensembles with admission while others run (serving).  pv_batch_admit places one
  * run (DOPRI5 from t=0 to t_end) if its vortices fit; *slot is -1 when they do not.

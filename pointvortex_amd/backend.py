@@ -149,6 +149,8 @@ _SIGNATURES = {
     "pv_batch_rk4": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]),
     "pv_batch_read": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_size_t]),
     "pv_batch_evaluate": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_size_t]),
+    "pv_batch_failures": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint),
+                                         ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_size_t]),
     "pv_batch_admit": (ctypes.c_int, [_vp, ctypes.POINTER(PvInstanceConfig), _dp, _dp, _dp,
                                       ctypes.c_size_t] + [ctypes.c_double] * 6 +
                        [ctypes.POINTER(ctypes.c_int)]),

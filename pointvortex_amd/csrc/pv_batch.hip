@@ -37,8 +37,15 @@ __device__ void batch_item(const BatchLaunch &a, const Instance *ins, const Work
     const int ti = it.tbegin + lane;
     const bool valid = ti < it.tend;
     const int gi = off + (valid ? ti : it.tbegin);
-    InstanceStep *stp = ADAPT ? &a.steps[it.instance] : nullptr;
-    const FailSink fail{a.st, a.epoch, ADAPT ? &stp->failure : nullptr};
+    InstanceStep *stp = &a.steps[it.instance];
+    FailSink fail{a.st, a.epoch, &stp->failure};
+    if (!ADAPT) {
+        fail.keep = stp->keep;
+        fail.x0 = a.op.x0;
+        fail.y0 = a.op.y0;
+        fail.steps_at_failure = &stp->fail_step;
+        fail.steps_done = a.counter + 1;
+    }
     const double xt = a.x[gi], yt = a.y[gi];
     if (valid) {
         if (!finite64(xt) || !finite64(yt))
@@ -105,8 +112,6 @@ __device__ void batch_item(const BatchLaunch &a, const Instance *ins, const Work
 
 template <class PI, class PP, class PD, bool ADAPT>
 __global__ __launch_bounds__(kBatchBlock) void batch_kernel(BatchLaunch a) {
-    if (!ADAPT && a.st->fail_epoch < a.epoch)
-        return;
     constexpr size_t kSrcBytes =
         sizeof(typename PD::Src) > sizeof(typename PI::Src)
             ? (sizeof(typename PD::Src) > sizeof(typename PP::Src) ? sizeof(typename PD::Src)
@@ -125,13 +130,13 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(BatchLaunch a) {
             break;
         const WorkItem it = a.items[item];
         const Instance *ins = &a.inst[it.instance];
-        if (ADAPT) {
-            // A run that failed in an earlier pass of this attempt skips the rest, so its
-            // failure word holds what the first failing evaluation found.
-            const InstanceStep &sp = a.steps[it.instance];
-            if (sp.state != kRunning || sp.failure || (a.pass == kPassNeedK0 && sp.fsal))
-                continue;
-        }
+        // An instance that failed in an earlier launch skips the rest, so its failure word
+        // holds what the first failing evaluation found and its state stays where it was.
+        const InstanceStep &sp = a.steps[it.instance];
+        if (sp.failure)
+            continue;
+        if (ADAPT && (sp.state != kRunning || (a.pass == kPassNeedK0 && sp.fsal)))
+            continue;
         const int geometry = ins->gp.geometry;
         if (geometry == 0)
             batch_item<PI, ADAPT>(a, ins, it, reinterpret_cast<typename PI::Src *>(mine), lane);
@@ -145,6 +150,12 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(BatchLaunch a) {
 __global__ void batch_reset_kernel(unsigned *counter) {
     if (threadIdx.x == 0)
         *counter = 0u;
+}
+
+// One fixed RK4 step finished (its last stage launch precedes this on the stream).
+__global__ void batch_step_done_kernel(unsigned *counter) {
+    if (threadIdx.x == 0)
+        counter[1] += 1u;
 }
 
 // DOPRI5 stage-1 positions with each vortex's own instance dt (owner[i] = instance index).
@@ -247,6 +258,10 @@ void launch_batch(int precision, bool adaptive, const BatchLaunch &a, int blocks
 
 void launch_batch_reset(unsigned *counter, hipStream_t s) {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, counter);
+}
+
+void launch_batch_step_done(unsigned *counter, hipStream_t s) {
+    hipLaunchKernelGGL(batch_step_done_kernel, dim3(1), dim3(64), 0, s, counter);
 }
 
 void launch_batch_stage1(const StageOp &op, const int *owner, const InstanceStep *steps, int n,

@@ -22,8 +22,10 @@ struct WorkItem {
     int pad;
 };
 
-// Per-instance adaptive DOPRI5 state (src/timestep.cpp:62-119 and the step landing of
-// src/main.cpp:312-314), resident on the device.
+// Per-instance state resident on the device: the adaptive DOPRI5 run (src/timestep.cpp:62-119
+// and the step landing of src/main.cpp:312-314), and for both modes the instance's failure
+// word.  A fixed-step instance that fails keeps the input state of its failing step at
+// keep[0..1] (one of the two ping-pong position buffers).
 struct InstanceStep {
     double t, dt, sugg, t_end;        // time, trial step, carried suggestion, horizon
     double atol, rtol, min_dt, max_dt;
@@ -34,7 +36,8 @@ struct InstanceStep {
     int accept_now;                   // set by the controller for the accept pass
     unsigned accepted, rejected_total, rejected_now;
     unsigned failure;                 // FailureKind bits (kFailRejected: too many rejections)
-    int pad;
+    unsigned fail_step;               // fixed-step mode: steps completed before the failing one
+    const double *keep[2];            // fixed-step mode: {x, y} before the failing step
 };
 
 enum InstanceState : int { kRunning = 0, kFinished = 1, kFailed = 2, kIdle = 3 };
@@ -49,7 +52,8 @@ struct BatchLaunch {
     const Instance *inst;
     const WorkItem *items;
     int nitems;
-    unsigned *counter;         // next item; zeroed before every launch
+    unsigned *counter;         // [0] next item, zeroed before every launch; [1] RK4 steps
+                               // completed since the ensemble was loaded
     const double *x, *y, *g;   // positions evaluated (flat) and circulations
     DevStatus *st;
     unsigned epoch;
@@ -61,6 +65,7 @@ struct BatchLaunch {
 // Launchers (pv_batch.hip).
 void launch_batch(int precision, bool adaptive, const BatchLaunch &a, int blocks, hipStream_t s);
 void launch_batch_reset(unsigned *counter, hipStream_t s);
+void launch_batch_step_done(unsigned *counter, hipStream_t s);
 void launch_batch_stage1(const StageOp &op, const int *owner, const InstanceStep *steps, int n,
                          hipStream_t s);
 void launch_batch_control(InstanceStep *steps, int ninst, hipStream_t s);

@@ -111,6 +111,7 @@ struct pv_batch {
     // fixed-step layout
     size_t total = 0;
     size_t nfixed = 0;
+    unsigned long long steps_done = 0;   // RK4 steps since pv_batch_load
     std::map<std::pair<double, bool>, hipGraphExec_t> graphs;  // (dt, orientation) -> 2 steps
 
     // adaptive layout
@@ -171,7 +172,7 @@ void launch(pv_batch *b, const double *px, const double *py, const pv::StageOp &
     a.st = b->st;
     a.epoch = ++b->epoch;
     a.op = op;
-    a.steps = adaptive ? b->steps_d : nullptr;
+    a.steps = b->steps_d;
     a.pass = pass;
     pv::launch_batch(b->precision, adaptive, a, batch_blocks(b), b->stream);
     hip_check(hipGetLastError(), "launch HIP ensemble kernel");
@@ -218,19 +219,42 @@ void read_status(pv_batch *b) {
     hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
 }
 
+// Fixed-step calls fail per instance (the instance stops; the others go on): refresh the
+// host mirror of the instance words once the stream is done.
 void finish_fixed(pv_batch *b) {
-    read_status(b);
-    const unsigned f = b->st_host->flags;
-    if (!f)
-        return;
-    if (f & pv::kFailStateNonFinite)
-        raise(PV_ERR_INVALID_ARGUMENT, "ensemble: vortex data must be finite");
-    if (f & pv::kFailOutsideDisk)
-        raise(PV_ERR_INVALID_ARGUMENT, "ensemble: vortex lies on or outside the disk");
-    if (f & pv::kFailVelocity)
-        raise(PV_ERR_RUNTIME,
-              "ensemble: non-finite velocity (coincident vortices or overflow) in an instance");
-    raise(PV_ERR_RUNTIME, "ensemble: evaluation failed");
+    if (b->nfixed)
+        hip_check(hipMemcpyAsync(b->steps_h, b->steps_d, b->nfixed * sizeof(pv::InstanceStep),
+                                 hipMemcpyDeviceToHost, b->stream),
+                  "read instance states");
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+}
+
+// Every instance slot idle, no vortex owned, the whole capacity free.
+void clear_instances(pv_batch *b) {
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    for (size_t k = 0; k < b->max_inst; ++k) {
+        std::memset(&b->steps_h[k], 0, sizeof(pv::InstanceStep));
+        b->steps_h[k].state = pv::kIdle;
+    }
+    hip_check(hipMemcpyAsync(b->steps_d, b->steps_h, b->max_inst * sizeof(pv::InstanceStep),
+                             hipMemcpyHostToDevice, b->stream),
+              "upload instance states");
+    std::vector<int> minus(b->capacity, -1);
+    hip_check(hipMemcpyAsync(b->owner, minus.data(), b->capacity * sizeof(int),
+                             hipMemcpyHostToDevice, b->stream),
+              "clear owners");
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    b->used.assign(b->max_inst, 0);
+    b->in_items.assign(b->max_inst, 0);
+    b->free_ranges = {{0, b->capacity}};
+    b->total = 0;
+    b->nfixed = 0;
+    b->nitems = 0;
+    b->items_dirty = true;
+    b->steps_done = 0;
+    const unsigned zero[2] = {0u, 0u};
+    hip_check(hipMemcpy(b->counter, zero, sizeof zero, hipMemcpyHostToDevice),
+              "clear step count");
 }
 
 // One RK4 step of every instance from (x0, y0) into (x1, y1) (src/timestep.cpp:31-61).
@@ -253,6 +277,8 @@ void enqueue_rk4(pv_batch *b, double dt, const double *x0, const double *y0, dou
     op.kind = pv::kOpRk4Final, op.cur = 3;
     op.ku = b->kx[3], op.kv = b->ky[3], op.xn = x1, op.yn = y1;
     launch(b, b->xa, b->ya, op, false, pv::kPassAll);
+    pv::launch_batch_step_done(b->counter, b->stream);
+    hip_check(hipGetLastError(), "launch HIP ensemble step count");
 }
 
 void upload_slot(pv_batch *b, int k) {
@@ -341,7 +367,7 @@ pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max
         std::vector<int> minus(n, -1);
         hip_check(hipMemcpy(b->owner, minus.data(), n * sizeof(int), hipMemcpyHostToDevice),
                   "clear owners");
-        b->counter = dalloc<unsigned>(1, "hipMalloc(work counter)");
+        b->counter = dalloc<unsigned>(2, "hipMalloc(work counter)");
         b->st = dalloc<pv::DevStatus>(1, "hipMalloc(status)");
         hip_check(hipHostMalloc((void **)&b->st_host, sizeof(pv::DevStatus), hipHostMallocDefault),
                   "hipHostMalloc(status)");
@@ -390,7 +416,7 @@ pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const si
         if (total > b->capacity)
             raise(PV_ERR_INVALID_ARGUMENT, "more vortices than the ensemble was created for");
         DeviceGuard dg(b->device);
-        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        clear_instances(b);
         size_t off = 0;
         std::vector<int> owner(total);
         std::vector<int> all;
@@ -401,6 +427,7 @@ pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const si
             b->inst_h[k].gp = geo_params(c);
             b->inst_h[k].offset = (int)off;
             b->inst_h[k].n = (int)counts[k];
+            b->steps_h[k].state = pv::kRunning;
             std::fill(owner.begin() + off, owner.begin() + off + counts[k], (int)k);
             if (counts[k])
                 all.push_back((int)k);
@@ -417,6 +444,9 @@ pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const si
         hip_check(hipMemcpyAsync(b->inst_d, b->inst_h.data(), ninst * sizeof(pv::Instance),
                                  hipMemcpyHostToDevice, b->stream),
                   "copy instances");
+        hip_check(hipMemcpyAsync(b->steps_d, b->steps_h, ninst * sizeof(pv::InstanceStep),
+                                 hipMemcpyHostToDevice, b->stream),
+                  "upload instance states");
         build_items(b, all);
         b->total = total;
         b->nfixed = ninst;
@@ -474,6 +504,7 @@ pv_status pv_batch_rk4(pv_batch *b, double dt, int steps, int use_graph) {
             std::swap(b->x, b->xn);
             std::swap(b->y, b->yn);
         }
+        b->steps_done += (unsigned long long)steps;
         finish_fixed(b);
     });
 }
@@ -491,7 +522,37 @@ pv_status pv_batch_read(pv_batch *b, double *x, double *y, size_t total) {
         hip_check(hipMemcpyAsync(y, b->y, total * sizeof(double), hipMemcpyDeviceToHost,
                                  b->stream),
                   "download y");
+        // A failed instance was left untouched from its failing step on: its state is the
+        // input of that step, in whichever position buffer the step read.
+        for (size_t k = 0; k < b->nfixed; ++k) {
+            const pv::InstanceStep &s = b->steps_h[k];
+            if (!s.failure || !s.keep[0] || s.keep[0] == b->x)
+                continue;
+            const size_t off = (size_t)b->inst_h[k].offset, n = (size_t)b->inst_h[k].n;
+            hip_check(hipMemcpyAsync(x + off, s.keep[0] + off, n * sizeof(double),
+                                     hipMemcpyDeviceToHost, b->stream),
+                      "download x");
+            hip_check(hipMemcpyAsync(y + off, s.keep[1] + off, n * sizeof(double),
+                                     hipMemcpyDeviceToHost, b->stream),
+                      "download y");
+        }
         hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    });
+}
+
+pv_status pv_batch_failures(pv_batch *b, unsigned *failures, unsigned long long *steps_done,
+                            size_t ninst) {
+    return guard([&] {
+        checked(b);
+        require_fixed(b);
+        if (ninst != b->nfixed || (ninst && !failures))
+            raise(PV_ERR_INVALID_ARGUMENT, "instance count differs from the loaded ensemble");
+        for (size_t k = 0; k < ninst; ++k) {
+            const pv::InstanceStep &s = b->steps_h[k];
+            failures[k] = s.failure;
+            if (steps_done)
+                steps_done[k] = s.failure ? s.fail_step : b->steps_done;
+        }
     });
 }
 
@@ -508,6 +569,7 @@ pv_status pv_batch_evaluate(pv_batch *b, double *u, double *v, size_t total) {
         pv::StageOp op{};
         op.kind = pv::kOpStore;
         op.ku = b->u, op.kv = b->v;
+        op.x0 = b->x, op.y0 = b->y;   // where a failing instance's state stays
         launch(b, b->x, b->y, op, false, pv::kPassAll);
         hip_check(hipMemcpyAsync(u, b->u, total * sizeof(double), hipMemcpyDeviceToHost,
                                  b->stream),
@@ -516,9 +578,6 @@ pv_status pv_batch_evaluate(pv_batch *b, double *u, double *v, size_t total) {
                                  b->stream),
                   "download v");
         finish_fixed(b);
-        for (size_t i = 0; i < total; ++i)
-            if (!std::isfinite(u[i]) || !std::isfinite(v[i]))
-                raise(PV_ERR_RUNTIME, "non-finite velocity; check scales and close encounters");
     });
 }
 
@@ -540,9 +599,9 @@ pv_status pv_batch_admit(pv_batch *b, const pv_instance_config *config, const do
             raise(PV_ERR_INVALID_ARGUMENT, "timestep must be finite and positive");
         if (!b->adaptive) {
             // First admission turns a loaded fixed-step ensemble into an adaptive one.
+            DeviceGuard dg(b->device);
+            clear_instances(b);
             b->adaptive = true;
-            b->total = 0;
-            b->nitems = 0;
         }
         int k = -1;
         for (size_t s = 0; s < b->max_inst; ++s)

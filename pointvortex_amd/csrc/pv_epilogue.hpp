@@ -13,17 +13,30 @@ __device__ __forceinline__ void record_failure(DevStatus *st, unsigned epoch, un
     atomicMin(&st->fail_epoch, epoch);
 }
 
-// Where a failure goes: the sticky status of the launch, or (adaptive ensembles) the failing
-// instance's own word, so that one failing run does not stop the others.
+// Where a failure goes: the sticky status of the launch, or (ensembles) the failing
+// instance's own word, so that one failing run does not stop the others.  For fixed-step
+// ensembles the first failure of an instance also records where the input state of the
+// failing step lives (keep = {x0, y0}): the instance is skipped from then on, so that state
+// stays intact in one of the two ping-pong buffers.
 struct FailSink {
     DevStatus *st;
     unsigned epoch;
     unsigned *word;
+    const double **keep = nullptr;     // [2]: where to record {x0, y0}, or null
+    const double *x0 = nullptr, *y0 = nullptr;
+    unsigned *steps_at_failure = nullptr;   // receives *steps_done at the first failure
+    const unsigned *steps_done = nullptr;
     __device__ __forceinline__ void operator()(unsigned kind) const {
-        if (word)
-            atomicOr(word, kind);
-        else
+        if (word) {
+            const unsigned old = atomicOr(word, kind);
+            if (old == 0u && keep) {
+                keep[0] = x0;
+                keep[1] = y0;
+                *steps_at_failure = *steps_done;
+            }
+        } else {
             record_failure(st, epoch, kind);
+        }
     }
 };
 

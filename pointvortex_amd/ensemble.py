@@ -114,6 +114,20 @@ class Ensemble:
         _check(self.lib.pv_batch_evaluate(self._h, _ptr(u), _ptr(v), total))
         return self._split(u), self._split(v)
 
+    def failures(self) -> List[str]:
+        """Per instance: '' when healthy, else the reason it stopped (it then holds the
+        state it had before its failing step)."""
+        return [f for f, _ in self.instance_status()]
+
+    def instance_status(self):
+        """Per instance: (failure reason or '', RK4 steps completed since load)."""
+        k = len(self.counts)
+        arr = (ctypes.c_uint * max(1, k))()
+        done = (ctypes.c_ulonglong * max(1, k))()
+        _check(self.lib.pv_batch_failures(self._h, arr, done, k))
+        return [(self.lib.pv_batch_failure_message(arr[i]).decode() if arr[i] else "",
+                 int(done[i])) for i in range(k)]
+
     # -- adaptive runs
     def admit(self, config: InstanceConfig, x, y, g, t_end: float, dt0: float = 1e-3,
               atol: float = 1e-10, rtol: float = 1e-8, min_dt: float = 1e-12,

@@ -105,7 +105,10 @@ class Server:
         return self.engines[precision]
 
     def run(self, requests: List[Request], clock=time.perf_counter,
-            max_wall: float = float("inf")) -> ServerStats:
+            max_wall: float = float("inf"), on_result=None) -> ServerStats:
+        """Serves `requests` (arrival times relative to the start) until all are done or
+        `max_wall` seconds have passed; `on_result(result, stats)` is called as each run
+        leaves."""
         pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         active: Dict[tuple, tuple] = {}
         stats = ServerStats()
@@ -149,6 +152,8 @@ class Server:
                     eng.release(s.slot)
                     stats.results.append(Result(req.rid, s.state, req.arrival, admitted, now,
                                                 s.accepted, s.rejected, s.failure, prec, x, y))
+                    if on_result is not None:
+                        on_result(stats.results[-1], stats)
         stats.wall = clock() - t0
         return stats
 
@@ -156,14 +161,16 @@ class Server:
 def make_requests(count: int, seed: int, rate: float, n_min: int = 64, n_max: int = 131072,
                   t_min: float = 0.1, t_max: float = 1.0, rtol_min: float = 1e-10,
                   rtol_max: float = 1e-6, atol_factor: float = 1e-3) -> List[Request]:
-    """Open-loop Poisson arrivals at `rate` per second; geometry uniform over the three;
-    N log-uniform in [n_min, n_max] (even); rtol log-uniform; horizon uniform; atol =
-    atol_factor * rtol; each request seeded by its index."""
+    """Open-loop Poisson arrivals at `rate` per second (all at t=0 when rate is 0); geometry
+    uniform over the three; N log-uniform in [n_min, n_max] (even); rtol log-uniform; horizon
+    uniform; atol = atol_factor * rtol; each request seeded by its index.  Arrival times come
+    from their own random stream, so the requests themselves do not depend on the rate."""
     rng = np.random.default_rng(seed)
+    gaps = np.random.default_rng([seed, 1]).standard_exponential(count)
     t = 0.0
     out = []
     for k in range(count):
-        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        t += gaps[k] / rate if rate > 0 else 0.0
         geo = ["infinite", "periodic", "disk"][int(rng.integers(0, 3))]
         n = int(np.floor(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
         n = max(2, n - n % 2)

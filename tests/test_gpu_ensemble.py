@@ -233,3 +233,49 @@ def test_serving_loop_completes_and_matches_alone():
         r = alone.run([req0], max_wall=30.0).results[0]
         assert r.accepted == by_id[req.rid].accepted
         assert np.array_equal(r.x, by_id[req.rid].x) and np.array_equal(r.y, by_id[req.rid].y)
+
+
+def test_fixed_step_failures_are_per_instance(oracle):
+    """A fixed-step instance that fails stops at the input state of its failing step and the
+    others go on unchanged: a coincident pair fails in the first step, a dipole driven into
+    the disk wall in the seventh.  Nine steps run as four two-step graph replays plus one
+    eager step, so both stopped states end up in the buffer the last step swapped away."""
+    cfgs, pops = mixed_population(6, 4)
+    healthy = [(c, p.x, p.y, p.circulation) for c, p in zip(cfgs, pops)]
+    good = wl.infinite(40, 9)
+    bx, by = good.x.copy(), good.y.copy()
+    bx[3], by[3] = bx[7], by[7]
+    coincident = (InstanceConfig("infinite"), bx, by, good.circulation)
+    wx, wy = np.array([0.8, 0.8, -0.3, -0.2]), np.array([0.005, -0.005, 0.1, -0.4])
+    wg = np.array([1.0, -1.0, 0.5, -0.5])
+    wall = (InstanceConfig("disk"), wx, wy, wg)
+    members = healthy[:3] + [coincident] + healthy[3:] + [wall]
+    e = Ensemble(sum(len(m[1]) for m in members), 16)
+    e.load(*[list(t) for t in zip(*members)])
+    ref = Ensemble(sum(len(m[1]) for m in healthy), 16)
+    ref.load(*[list(t) for t in zip(*healthy)])
+    dt = 2e-3
+    e.rk4(dt, 9, use_graph=True)
+    ref.rk4(dt, 9, use_graph=True)
+    status = e.instance_status()
+    fails = [f for f, _ in status]
+    assert [d for _, d in status] == [9, 9, 9, 0, 9, 9, 9, 6], status
+    assert fails[3] == "non-finite velocity; check scales and close encounters", fails
+    assert fails[7] == "vortex lies on or outside the disk", fails
+    assert all(f == "" for k, f in enumerate(fails) if k not in (3, 7)), fails
+    xs, ys = e.read()
+    xr, yr = ref.read()
+    for k_e, k_r in zip([0, 1, 2, 4, 5, 6], range(6)):
+        assert np.array_equal(xs[k_e], xr[k_r]) and np.array_equal(ys[k_e], yr[k_r])
+    assert np.array_equal(xs[3], bx) and np.array_equal(ys[3], by)
+    # The wall instance's state after six steps, against the oracle; the tolerance is the
+    # oracle's own response to a 1e-12 relative velocity change over those steps.
+    spec_disk = ospec(wall[0])
+    ox, oy = oracle_rk4(oracle, wall[0], wx, wy, wg, dt, 6)
+    px, py = oracle_rk4(oracle, wall[0], wx, wy, wg * (1 + 1e-12 * np.array([1, -1, 1, -1])),
+                        dt, 6)
+    amplified = normwise(px - wx, py - wy, ox - wx, oy - wy)
+    err = normwise(xs[7] - wx, ys[7] - wy, ox - wx, oy - wy)
+    assert err <= max(1e-10, 10 * amplified), (err, amplified)
+    with pytest.raises(Exception, match="outside the disk"):
+        oracle.integrator(4).rk4_step(spec_disk, ox.copy(), oy.copy(), wg, dt)
