@@ -32,6 +32,21 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 
 
+TRAFFIC_SUMMARY = ROOT / "profiles" / "pair_kernel_disk_f64_n65536_hbm_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP64 disk pair kernel at N=65536 from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), or None when absent."""
+    try:
+        for line in TRAFFIC_SUMMARY.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -321,7 +336,8 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic() if n == 65536 else None,
+            "traffic_source": str(TRAFFIC_SUMMARY.relative_to(ROOT)),
             "kernel": "pair_kernel<DiskF64>",
             "flops_per_pair": FLOPS_PER_PAIR["disk"],
             "avg_launch_ms": avg_ms,

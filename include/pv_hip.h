@@ -161,9 +161,30 @@ pv_status pv_runtime_details(int device, char *buffer, size_t length);
 pv_status pv_evaluate_resident(pv_handle *h, int repeats);
 pv_status pv_synchronize(pv_handle *h);
 
-/* Launch-shape override for the pair kernels (no reference counterpart): targets per
- * thread (1, 2 or 4) and number of source splits (0 = automatic).  Used by the autotuner. */
+/* Launch shapes of the pair kernels (no reference counterpart; the CUDA backend used one
+ * fixed shape, src/backend_cuda.cu:402-416).  A shape is targets per thread (1, 2 or 4; 0 =
+ * automatic), source splits (0 = automatic) and the LDS source tile (128 or 256; 0 = the
+ * precision's default).  Precedence: pv_set_launch / pv_set_launch_shape, then the first
+ * matching rule of the handle's launch table, then a built-in heuristic.  A new handle's
+ * table is read from the file named by $PV_HIP_LAUNCH_TABLE, else from launch_table.txt
+ * beside libpvhip.so when present (lines "geometry precision n_max tpt splits tile", as
+ * written by the autotuner, pointvortex_amd/autotune.py).  pv_resolve_launch reports the
+ * shape an evaluation of `targets` targets against n sources would use. */
+typedef struct pv_launch_rule {
+    int geometry;            /* pv_geometry */
+    int precision;           /* pv_precision */
+    size_t n_max;            /* applies to populations of at most n_max vortices */
+    int targets_per_thread;
+    int source_splits;
+    int tile;
+} pv_launch_rule;
+
 pv_status pv_set_launch(pv_handle *h, int targets_per_thread, int source_splits);
+pv_status pv_set_launch_shape(pv_handle *h, int targets_per_thread, int source_splits,
+                              int tile);
+pv_status pv_set_launch_table(pv_handle *h, const pv_launch_rule *rules, size_t count);
+pv_status pv_resolve_launch(pv_handle *h, size_t targets, size_t n, int *targets_per_thread,
+                            int *source_splits, int *tile);
 
 /* Kernel timing for benchmarks (no reference counterpart): between begin and end every
  * velocity-kernel launch of the handle is bracketed by HIP events recorded on the handle's own

@@ -1,0 +1,195 @@
+"""Autotuner of the pair-kernel launch shape per (geometry, precision, N-bucket).
+
+A shape is (targets per thread, source splits, LDS source tile); its blocks per CU follow from
+the split count.  For every geometry, precision and N-bucket the tuner uploads a seeded
+population at the bucket's upper N, evaluates it once with the built-in shape (the
+reference result), then for every distinct candidate shape:
+
+* parity gate first: the candidate's velocities must match the built-in shape's to the
+  product's own bar -- normwise 1e-12 in FP64; in mixed precision, within 1e-6 of the FP64
+  kernel and no more than 1.5x the built-in mixed shape's own distance from it;
+* then timing: kernel time from HIP events on the handle's stream, averaged over enough
+  launches to cover about 30 ms.
+
+The fastest shape that passed the gate wins.  The table is text, one rule per line:
+``geometry precision n_max targets_per_thread source_splits tile`` ('#' comments), the format
+libpvhip.so reads from $PV_HIP_LAUNCH_TABLE or from launch_table.txt beside the library
+(pointvortex_amd.build installs pointvortex_amd/tuning/launch_table.txt there).
+
+Usage: python -m pointvortex_amd.autotune [--out FILE] [--geometries ...] [--buckets ...]
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import math
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import workloads as wl
+from .backend import HipKernel, SimParams, VortexSystem
+
+BUCKETS = (256, 1024, 4096, 16384, 65536, 262144)
+TPT = (1, 2, 4)
+TILES = (128, 256)
+SPLITS = (0, 1, 2, 4, 8, 16, 32, 64)
+GEOMETRIES = ("infinite", "periodic", "disk")
+PRECISIONS = ("fp64", "mixed")
+SIZE_MAX = 2 ** 64 - 1
+TABLE = Path(__file__).resolve().parent / "tuning" / "launch_table.txt"
+
+FP64_GATE = 1e-12
+MIXED_GATE = 1e-6
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def normwise(u, v, ur, vr):
+    den = math.sqrt(float(np.sum(ur ** 2 + vr ** 2)))
+    num = math.sqrt(float(np.sum((u - ur) ** 2 + (v - vr) ** 2)))
+    return num / den if den > 0 else num
+
+
+def population(geometry, n, seed=11):
+    n += n % 2
+    if geometry == "infinite":
+        return wl.infinite(n, seed)
+    if geometry == "periodic":
+        return wl.periodic(n, seed)
+    return wl.disk(n, seed)
+
+
+def _kernel(geometry, precision, pop):
+    k = HipKernel(SimParams(boundaryCondition=geometry, boxLengthX=2 * np.pi,
+                            boxLengthY=2 * np.pi, diskRadius=1.0, precision=precision))
+    k.set_launch_table([])
+    k.uploadDeviceState(VortexSystem(pop.x, pop.y, pop.circulation))
+    return k
+
+
+def _time(k, budget_s=0.03):
+    k.evaluateResident(1)
+    k.synchronize()
+    k.profileBegin()
+    k.evaluateResident(1)
+    k.synchronize()
+    ms, launches = k.profileEnd()
+    one = ms / launches
+    reps = max(1, min(50, int(budget_s * 1e3 / max(one, 1e-3))))
+    k.profileBegin()
+    k.evaluateResident(reps)
+    k.synchronize()
+    ms, launches = k.profileEnd()
+    return ms / launches
+
+
+def tune_one(geometry, precision, n, log=_log):
+    """Best gated shape for one (geometry, precision, N); returns a result dict."""
+    pop = population(geometry, n)
+    n = pop.n
+    k = _kernel(geometry, precision, pop)
+    base = k.evaluateDeviceState()
+    default_shape = k.resolved_launch(n, n)
+    default_ms = _time(k)
+    ref64, base_err = base, 0.0
+    if precision == "mixed":
+        k64 = _kernel(geometry, "fp64", pop)
+        ref64 = k64.evaluateDeviceState()
+        k64.close()
+        base_err = normwise(base.x, base.y, ref64.x, ref64.y)
+    best_shape, best_ms = default_shape, default_ms
+    seen = {default_shape}
+    rejected = 0
+    for tpt, tile, splits in itertools.product(TPT, TILES, SPLITS):
+        k.set_launch(tpt, splits, tile)
+        shape = k.resolved_launch(n, n)
+        if shape in seen:
+            continue
+        seen.add(shape)
+        vel = k.evaluateDeviceState()
+        if precision == "fp64":
+            ok = normwise(vel.x, vel.y, base.x, base.y) <= FP64_GATE
+        else:
+            err = normwise(vel.x, vel.y, ref64.x, ref64.y)
+            ok = err <= MIXED_GATE and err <= max(1.5 * base_err, 1e-7)
+        if not ok:
+            rejected += 1
+            continue
+        ms = _time(k)
+        if ms < best_ms:
+            best_shape, best_ms = shape, ms
+    k.close()
+    res = {"geometry": geometry, "precision": precision, "n": n,
+           "default": default_shape, "default_ms": default_ms,
+           "best": best_shape, "best_ms": best_ms, "candidates": len(seen),
+           "rejected": rejected}
+    log(f"[autotune] {geometry:8s} {precision:5s} N={n:7d}: default {default_shape} "
+        f"{default_ms:.4f} ms -> {best_shape} {best_ms:.4f} ms "
+        f"({len(seen)} shapes, {rejected} failed the gate)")
+    return res
+
+
+def tune(geometries=GEOMETRIES, precisions=PRECISIONS, buckets=BUCKETS, log=_log):
+    out = []
+    for geometry, precision in itertools.product(geometries, precisions):
+        for n in buckets:
+            out.append(tune_one(geometry, precision, n, log))
+    return out
+
+
+def table_lines(results):
+    """Rules in table order; the largest bucket of each (geometry, precision) also covers
+    every larger N."""
+    lines = ["# geometry precision n_max targets_per_thread source_splits tile"]
+    groups = {}
+    for r in results:
+        groups.setdefault((r["geometry"], r["precision"]), []).append(r)
+    for (geometry, precision), rs in groups.items():
+        rs = sorted(rs, key=lambda r: r["n"])
+        for i, r in enumerate(rs):
+            n_max = SIZE_MAX if i == len(rs) - 1 else r["n"]
+            tpt, splits, tile = r["best"]
+            lines.append(f"{geometry} {precision} {n_max} {tpt} {splits} {tile}  "
+                         f"# N={r['n']}: {r['default_ms']:.4f} ms built-in {r['default']}, "
+                         f"{r['best_ms']:.4f} ms tuned")
+    return lines
+
+
+def read_table(path):
+    """Rules of a table file as tuples (geometry, precision, n_max, tpt, splits, tile)."""
+    rules = []
+    for line in Path(path).read_text().splitlines():
+        line = line.split("#", 1)[0].split()
+        if not line:
+            continue
+        if len(line) != 6:
+            raise ValueError(f"malformed launch rule: {' '.join(line)}")
+        g, p, n, t, s, tl = line
+        if g not in GEOMETRIES or p not in PRECISIONS:
+            raise ValueError(f"unknown geometry or precision in launch rule: {' '.join(line)}")
+        rules.append((g, p, int(n), int(t), int(s), int(tl)))
+    return rules
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default=str(TABLE))
+    ap.add_argument("--geometries", nargs="+", default=list(GEOMETRIES))
+    ap.add_argument("--precisions", nargs="+", default=list(PRECISIONS))
+    ap.add_argument("--buckets", nargs="+", type=int, default=list(BUCKETS))
+    args = ap.parse_args()
+    t0 = time.perf_counter()
+    results = tune(args.geometries, args.precisions, args.buckets)
+    lines = table_lines(results)
+    Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+    Path(args.out).write_text("\n".join(lines) + "\n")
+    _log(f"[autotune] wrote {args.out} ({len(lines) - 1} rules, {time.perf_counter() - t0:.0f} s)")
+
+
+if __name__ == "__main__":
+    main()

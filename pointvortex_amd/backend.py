@@ -89,6 +89,17 @@ class PvInstanceConfig(ctypes.Structure):
     ]
 
 
+class PvLaunchRule(ctypes.Structure):
+    _fields_ = [
+        ("geometry", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("n_max", ctypes.c_size_t),
+        ("targets_per_thread", ctypes.c_int),
+        ("source_splits", ctypes.c_int),
+        ("tile", ctypes.c_int),
+    ]
+
+
 class PvSlotState(ctypes.Structure):
     _fields_ = [
         ("slot", ctypes.c_int),
@@ -136,6 +147,12 @@ _SIGNATURES = {
     "pv_evaluate_resident": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pv_synchronize": (ctypes.c_int, [_vp]),
     "pv_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "pv_set_launch_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pv_set_launch_table": (ctypes.c_int, [_vp, ctypes.POINTER(PvLaunchRule), ctypes.c_size_t]),
+    "pv_resolve_launch": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int)]),
     "pv_profile_begin": (ctypes.c_int, [_vp]),
     "pv_profile_end": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "pv_calibration_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp]),
@@ -293,8 +310,26 @@ class HipKernel:
         _check(_lib.pv_set_precision(self._h, PRECISIONS[precision]))
         self.params.precision = precision
 
-    def set_launch(self, targets_per_thread=0, source_splits=0):
-        _check(_lib.pv_set_launch(self._h, int(targets_per_thread), int(source_splits)))
+    def set_launch(self, targets_per_thread=0, source_splits=0, tile=0):
+        """Explicit launch shape of the pair kernels (0 = automatic / default)."""
+        _check(_lib.pv_set_launch_shape(self._h, int(targets_per_thread), int(source_splits),
+                                        int(tile)))
+
+    def set_launch_table(self, rules):
+        """rules: iterable of (geometry, precision, n_max, targets_per_thread, source_splits,
+        tile); the first rule matching the handle's geometry, precision and N applies."""
+        rules = list(rules)
+        arr = (PvLaunchRule * max(1, len(rules)))(
+            *[PvLaunchRule(GEOMETRIES[g], PRECISIONS[p], int(n), int(t), int(s), int(tl))
+              for g, p, n, t, s, tl in rules])
+        _check(_lib.pv_set_launch_table(self._h, arr, len(rules)))
+
+    def resolved_launch(self, targets, n):
+        """(targets_per_thread, source_splits, tile) an evaluation would use."""
+        t, s, tl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(_lib.pv_resolve_launch(self._h, int(targets), int(n), ctypes.byref(t),
+                                      ctypes.byref(s), ctypes.byref(tl)))
+        return t.value, s.value, tl.value
 
     # -- pure evaluation (src/compute.h:21-27)
     def evaluateRange(self, x, y, circulation, begin, end, velocity: Optional[VectorField] = None):

@@ -19,6 +19,8 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libpvhip.so"
+# Tuned launch shapes (pointvortex_amd.autotune); the library reads launch_table.txt beside it.
+TUNED = PKG / "tuning" / "launch_table.txt"
 INCLUDE = ROOT / "include"
 
 ARCH = os.environ.get("PV_OFFLOAD_ARCH", "gfx950")
@@ -58,6 +60,11 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    installed = LIBDIR / "launch_table.txt"
+    if TUNED.exists():
+        shutil.copyfile(TUNED, installed)
+    elif installed.exists():
+        installed.unlink()
     return LIB
 
 

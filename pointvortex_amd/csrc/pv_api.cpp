@@ -11,11 +11,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <climits>
-#include <memory>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -77,7 +82,8 @@ struct pv_handle {
     pv::DevStatus *st = nullptr;
     pv::DevStatus *st_host = nullptr;
 
-    int tpt_override = 0, split_override = 0;
+    int tpt_override = 0, split_override = 0, tile_override = 0;
+    std::vector<pv_launch_rule> table;   // tuned shapes, first matching rule wins
     bool profiling = false;
     std::vector<hipEvent_t> events;   // pool, two per profiled launch
     size_t events_used = 0;
@@ -207,27 +213,142 @@ void begin_sequence(pv_handle *h) {
     h->records.clear();
 }
 
+struct Shape {
+    int tpt, tile;
+    size_t tiles, nsplit, split_len;
+};
+
+// Launch shape of an evaluation of `targets` targets against n sources.  Precedence: an
+// explicit pv_set_launch_shape, then the first matching rule of the handle's launch table,
+// then the built-in heuristic (2 targets per thread from 32768 targets; enough source splits
+// for about 4 blocks per CU).
+Shape resolve_shape(const pv_handle *h, size_t targets, size_t n) {
+    const int prec = h->cfg.precision;
+    int tpt = 0, splits = 0, tile = 0;
+    for (const pv_launch_rule &r : h->table) {
+        if (r.geometry == h->cfg.geometry && r.precision == prec && n <= r.n_max) {
+            tpt = r.targets_per_thread, splits = r.source_splits, tile = r.tile;
+            break;
+        }
+    }
+    if (h->tpt_override > 0)
+        tpt = h->tpt_override;
+    if (h->split_override > 0)
+        splits = h->split_override;
+    if (h->tile_override > 0)
+        tile = h->tile_override;
+    if (tpt <= 0)
+        tpt = targets >= 32768 ? 2 : 1;
+    tpt = tpt >= 4 ? 4 : (tpt >= 2 ? 2 : 1);
+    if (tile != 128 && tile != 256)
+        tile = pv::pair_tile(prec);
+    Shape s{};
+    s.tpt = tpt;
+    s.tile = tile;
+    s.tiles = (targets + (size_t)pv::kBlock * tpt - 1) / ((size_t)pv::kBlock * tpt);
+    const size_t max_split = std::max<size_t>(1, (n + tile - 1) / tile);
+    size_t nsplit = splits > 0 ? (size_t)splits : 0;
+    if (nsplit == 0) {
+        const size_t want = (size_t)4 * h->cu_count;
+        nsplit = std::max<size_t>(1, (want + s.tiles - 1) / std::max<size_t>(1, s.tiles));
+    }
+    nsplit = std::min(nsplit, max_split);
+    size_t split_len = (n + nsplit - 1) / nsplit;
+    split_len = std::max<size_t>(tile, (split_len + tile - 1) / tile * tile);
+    s.split_len = split_len;
+    s.nsplit = std::max<size_t>(1, (n + split_len - 1) / split_len);
+    return s;
+}
+
+// Launch table text: one rule per line, "geometry precision n_max targets_per_thread
+// source_splits tile" (geometry infinite|periodic|disk, precision fp64|mixed); '#' starts a
+// comment.  A malformed table is an error, not a silent fallback.
+std::vector<pv_launch_rule> read_launch_table(const std::string &path) {
+    std::ifstream in(path);
+    if (!in)
+        raise(PV_ERR_INVALID_ARGUMENT, "cannot read launch table " + path);
+    std::vector<pv_launch_rule> rules;
+    std::string line;
+    int lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos)
+            line.resize(hash);
+        std::istringstream ss(line);
+        std::string geo, prec;
+        if (!(ss >> geo))
+            continue;
+        pv_launch_rule r{};
+        unsigned long long n_max = 0;
+        if (!(ss >> prec >> n_max >> r.targets_per_thread >> r.source_splits >> r.tile))
+            raise(PV_ERR_INVALID_ARGUMENT,
+                  "launch table " + path + ": line " + std::to_string(lineno) + " is malformed");
+        r.geometry = geo == "infinite" ? PV_INFINITE
+                     : geo == "periodic" ? PV_PERIODIC
+                     : geo == "disk"     ? PV_DISK
+                                         : -1;
+        r.precision = prec == "fp64" ? PV_FP64 : prec == "mixed" ? PV_MIXED : -1;
+        r.n_max = (size_t)n_max;
+        if (r.geometry < 0 || r.precision < 0)
+            raise(PV_ERR_INVALID_ARGUMENT, "launch table " + path + ": line " +
+                                               std::to_string(lineno) +
+                                               " names an unknown geometry or precision");
+        rules.push_back(r);
+    }
+    return rules;
+}
+
+void check_rules(const pv_launch_rule *rules, size_t count) {
+    for (size_t k = 0; k < count; ++k) {
+        const pv_launch_rule &r = rules[k];
+        if (r.geometry < PV_INFINITE || r.geometry > PV_DISK ||
+            (r.precision != PV_FP64 && r.precision != PV_MIXED))
+            raise(PV_ERR_INVALID_ARGUMENT, "launch rule names an unknown geometry or precision");
+        if (r.targets_per_thread != 0 && r.targets_per_thread != 1 && r.targets_per_thread != 2 &&
+            r.targets_per_thread != 4)
+            raise(PV_ERR_INVALID_ARGUMENT, "targets per thread must be 0, 1, 2 or 4");
+        if (r.source_splits < 0)
+            raise(PV_ERR_INVALID_ARGUMENT, "source splits must be non-negative");
+        if (r.tile != 0 && r.tile != 128 && r.tile != 256)
+            raise(PV_ERR_INVALID_ARGUMENT, "source tile must be 0, 128 or 256");
+    }
+}
+
+// The default launch table of a new handle: $PV_HIP_LAUNCH_TABLE, else launch_table.txt
+// beside this library when present, else none (built-in heuristic).
+std::vector<pv_launch_rule> default_launch_table() {
+    if (const char *env = std::getenv("PV_HIP_LAUNCH_TABLE")) {
+        if (*env) {
+            auto rules = read_launch_table(env);
+            check_rules(rules.data(), rules.size());
+            return rules;
+        }
+    }
+    Dl_info info{};
+    if (dladdr((void *)&pv_create, &info) && info.dli_fname) {
+        std::string path(info.dli_fname);
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+               "/launch_table.txt";
+        std::ifstream probe(path);
+        if (probe) {
+            auto rules = read_launch_table(path);
+            check_rules(rules.data(), rules.size());
+            return rules;
+        }
+    }
+    return {};
+}
+
 // One velocity evaluation of positions (x, y) with circulations g, targets [tb, te).
 void evaluate(pv_handle *h, const double *x, const double *y, const double *g, size_t n,
               size_t tb, size_t te, const pv::StageOp &op) {
     const int prec = h->cfg.precision;
     const size_t targets = te - tb;
-    int tpt = h->tpt_override;
-    if (tpt <= 0)
-        tpt = targets >= 32768 ? 2 : 1;
-    tpt = tpt >= 4 ? 4 : (tpt >= 2 ? 2 : 1);
-    const int tile = pv::pair_tile(prec);
-    const size_t tiles = (targets + (size_t)pv::kBlock * tpt - 1) / ((size_t)pv::kBlock * tpt);
-    const size_t max_split = std::max<size_t>(1, (n + tile - 1) / tile);
-    size_t nsplit = h->split_override > 0 ? (size_t)h->split_override : 0;
-    if (nsplit == 0) {
-        const size_t want = (size_t)4 * h->cu_count;
-        nsplit = std::max<size_t>(1, (want + tiles - 1) / tiles);
-    }
-    nsplit = std::min(nsplit, max_split);
-    size_t split_len = (n + nsplit - 1) / nsplit;
-    split_len = std::max<size_t>(tile, (split_len + tile - 1) / tile * tile);
-    nsplit = std::max<size_t>(1, (n + split_len - 1) / split_len);
+    const Shape sh = resolve_shape(h, targets, n);
+    const int tpt = sh.tpt;
+    const size_t tiles = sh.tiles, nsplit = sh.nsplit, split_len = sh.split_len;
     if (nsplit > 1)
         ensure_partials(h, nsplit * 2 * n);
     ensure_tickets(h, tiles);
@@ -255,7 +376,7 @@ void evaluate(pv_handle *h, const double *x, const double *y, const double *g, s
         ev1 = h->events[h->events_used++];
         hip_check(hipEventRecord(ev0, h->stream), "record HIP event");
     }
-    pv::launch_pair(h->cfg.geometry, prec, tpt, a, h->gp, h->stream);
+    pv::launch_pair(h->cfg.geometry, prec, tpt, sh.tile, a, h->gp, h->stream);
     hip_check(hipGetLastError(), "launch HIP velocity kernel");
     if (ev1)
         hip_check(hipEventRecord(ev1, h->stream), "record HIP event");
@@ -452,6 +573,7 @@ pv_status pv_create(const pv_config *config, pv_handle **out) {
         hip_check(hipHostMalloc((void **)&h->st_host, sizeof(pv::DevStatus), hipHostMallocDefault),
                   "hipHostMalloc(status)");
         std::memset(h->st_host, 0, sizeof(pv::DevStatus));
+        h->table = default_launch_table();
         begin_sequence(h.get());
         hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
         *out = h.release();
@@ -488,6 +610,43 @@ pv_status pv_set_launch(pv_handle *h, int targets_per_thread, int source_splits)
             raise(PV_ERR_INVALID_ARGUMENT, "source splits must be non-negative");
         h->tpt_override = targets_per_thread;
         h->split_override = source_splits;
+        h->tile_override = 0;
+    });
+}
+
+pv_status pv_set_launch_shape(pv_handle *h, int targets_per_thread, int source_splits,
+                              int tile) {
+    return guard([&] {
+        checked(h);
+        const pv_launch_rule r{h->cfg.geometry, h->cfg.precision, 0, targets_per_thread,
+                               source_splits, tile};
+        check_rules(&r, 1);
+        h->tpt_override = targets_per_thread;
+        h->split_override = source_splits;
+        h->tile_override = tile;
+    });
+}
+
+pv_status pv_set_launch_table(pv_handle *h, const pv_launch_rule *rules, size_t count) {
+    return guard([&] {
+        checked(h);
+        if (count && !rules)
+            raise(PV_ERR_INVALID_ARGUMENT, "null argument");
+        check_rules(rules, count);
+        h->table.assign(rules, rules + count);
+    });
+}
+
+pv_status pv_resolve_launch(pv_handle *h, size_t targets, size_t n, int *targets_per_thread,
+                            int *source_splits, int *tile) {
+    return guard([&] {
+        checked(h);
+        if (!targets_per_thread || !source_splits || !tile)
+            raise(PV_ERR_INVALID_ARGUMENT, "null argument");
+        const Shape s = resolve_shape(h, targets, n);
+        *targets_per_thread = s.tpt;
+        *source_splits = (int)s.nsplit;
+        *tile = s.tile;
     });
 }
 

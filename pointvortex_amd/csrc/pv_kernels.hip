@@ -473,24 +473,33 @@ static void launch_pair_tpt(int tpt, const PairLaunch &a, const GeoParams &gp, h
         launch_pair_t<P, 1, TILE>(a, gp, s);
 }
 
+template <class P>
+static void launch_pair_tile(int tpt, int tile, const PairLaunch &a, const GeoParams &gp,
+                             hipStream_t s) {
+    if (tile == 128)
+        launch_pair_tpt<P, 128>(tpt, a, gp, s);
+    else
+        launch_pair_tpt<P, 256>(tpt, a, gp, s);
+}
+
 int pair_tile(int precision) { return precision ? kTileMixed : kTileF64; }
 
-void launch_pair(int geometry, int precision, int tpt, const PairLaunch &a, const GeoParams &gp,
-                 hipStream_t s) {
+void launch_pair(int geometry, int precision, int tpt, int tile, const PairLaunch &a,
+                 const GeoParams &gp, hipStream_t s) {
     if (precision == 0) {
         if (geometry == 0)
-            launch_pair_tpt<InfF64, kTileF64>(tpt, a, gp, s);
+            launch_pair_tile<InfF64>(tpt, tile, a, gp, s);
         else if (geometry == 1)
-            launch_pair_tpt<PerF64, kTileF64>(tpt, a, gp, s);
+            launch_pair_tile<PerF64>(tpt, tile, a, gp, s);
         else
-            launch_pair_tpt<DiskF64, kTileF64>(tpt, a, gp, s);
+            launch_pair_tile<DiskF64>(tpt, tile, a, gp, s);
     } else {
         if (geometry == 0)
-            launch_pair_tpt<InfMixed, kTileMixed>(tpt, a, gp, s);
+            launch_pair_tile<InfMixed>(tpt, tile, a, gp, s);
         else if (geometry == 1)
-            launch_pair_tpt<PerMixed, kTileMixed>(tpt, a, gp, s);
+            launch_pair_tile<PerMixed>(tpt, tile, a, gp, s);
         else
-            launch_pair_tpt<DiskMixed, kTileMixed>(tpt, a, gp, s);
+            launch_pair_tile<DiskMixed>(tpt, tile, a, gp, s);
     }
 }
 

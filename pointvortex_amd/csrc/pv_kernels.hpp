@@ -57,8 +57,9 @@ constexpr int kTileF64 = 256;
 constexpr int kTileMixed = 128;
 
 // Launchers (pv_kernels.hip).  `tpt` is 1, 2 or 4.
-void launch_pair(int geometry, int precision, int tpt, const PairLaunch &a, const GeoParams &gp,
-                 hipStream_t s);
+// tile: sources staged in LDS per step, 128 or 256 (pair_tile() gives each precision's default).
+void launch_pair(int geometry, int precision, int tpt, int tile, const PairLaunch &a,
+                 const GeoParams &gp, hipStream_t s);
 int pair_tile(int precision);
 void launch_make_stage(const StageOp &op, int n, const DevStatus *st, unsigned epoch,
                        hipStream_t s);
