@@ -70,6 +70,27 @@ def run_rk4(pv, pop, precision, steps, warmup, dt):
     return k, t1 - t0, kernel_ms, launches
 
 
+def per_step_error(pv, pop, steps, dt):
+    """The mixed RK4 run replayed step by step: after every step the mixed and the FP64
+    kernels evaluate the mixed run's current state; returns the normwise error per step."""
+    km = pv.HipKernel(pv.SimParams(boundaryCondition="disk", diskRadius=1.0, precision="mixed"))
+    kf = pv.HipKernel(pv.SimParams(boundaryCondition="disk", diskRadius=1.0))
+    s = pv.VortexSystem(pop.x.copy(), pop.y.copy(), pop.circulation)
+    km.uploadDeviceState(s)
+    errs = []
+    for step in range(steps):
+        km.deviceRk4Step(dt)
+        km.downloadDeviceState(s)
+        kf.uploadDeviceState(s)
+        got, ref = km.evaluateDeviceState(), kf.evaluateDeviceState()
+        errs.append(normwise(got.x, got.y, ref.x, ref.y))
+        if (step + 1) % 50 == 0:
+            log(f"[bench] per-step error: {step + 1}/{steps} steps, max so far {max(errs):.3e}")
+    km.close()
+    kf.close()
+    return errs
+
+
 def per_geometry(pv, n, repeats=3):
     """Evaluation-only pair-interactions/s for the three geometries at N (FP64 and mixed)."""
     from pointvortex_amd import workloads as wl
@@ -307,6 +328,7 @@ def main():
         got = kmx.evaluateDeviceState()
         errs[state] = normwise(got.x, got.y, ref.x, ref.y)
     log(f"[bench] mixed normwise error vs fp64: {errs}")
+    step_errs = per_step_error(pv, pop, args.steps, dt)
 
     avg_ms = kms64 / launches64
     achieved = n * (n - 1) * FLOPS_PER_PAIR["disk"] / (avg_ms / 1e3) / 1e12
@@ -350,6 +372,9 @@ def main():
             "speedup_vs_fp64": value_mx / value,
             "normwise_err_vs_fp64_initial": errs["initial"],
             "normwise_err_vs_fp64_final": errs["final"],
+            "per_step_err_vs_fp64": {"steps": len(step_errs), "max": max(step_errs),
+                                     "mean": float(np.mean(step_errs)),
+                                     "last": step_errs[-1]},
             "avg_launch_ms": kmsmx / launchesmx,
         },
     }

@@ -11,7 +11,10 @@ reference result), then for every distinct candidate shape:
 * then timing: kernel time from HIP events on the handle's stream, averaged over enough
   launches to cover about 30 ms.
 
-The fastest shape that passed the gate wins.  The table is text, one rule per line:
+Among the shapes that passed the gate, the one with the fewest source splits within
+`TIE` (2 %) of the fastest wins: each extra split writes and re-reads one partial sum per
+target, so a shape that is not measurably faster should not buy its HBM traffic.  The table
+is text, one rule per line:
 ``geometry precision n_max targets_per_thread source_splits tile`` ('#' comments), the format
 libpvhip.so reads from $PV_HIP_LAUNCH_TABLE or from launch_table.txt beside the library
 (pointvortex_amd.build installs pointvortex_amd/tuning/launch_table.txt there).
@@ -43,6 +46,7 @@ TABLE = Path(__file__).resolve().parent / "tuning" / "launch_table.txt"
 
 FP64_GATE = 1e-12
 MIXED_GATE = 1e-6
+TIE = 0.02
 
 
 def _log(*a):
@@ -102,7 +106,7 @@ def tune_one(geometry, precision, n, log=_log):
         ref64 = k64.evaluateDeviceState()
         k64.close()
         base_err = normwise(base.x, base.y, ref64.x, ref64.y)
-    best_shape, best_ms = default_shape, default_ms
+    timed = {default_shape: default_ms}
     seen = {default_shape}
     rejected = 0
     for tpt, tile, splits in itertools.product(TPT, TILES, SPLITS):
@@ -120,10 +124,12 @@ def tune_one(geometry, precision, n, log=_log):
         if not ok:
             rejected += 1
             continue
-        ms = _time(k)
-        if ms < best_ms:
-            best_shape, best_ms = shape, ms
+        timed[shape] = _time(k)
     k.close()
+    fastest = min(timed.values())
+    near = [s for s, ms in timed.items() if ms <= fastest * (1.0 + TIE)]
+    best_shape = min(near, key=lambda s: (s[1], timed[s]))
+    best_ms = timed[best_shape]
     res = {"geometry": geometry, "precision": precision, "n": n,
            "default": default_shape, "default_ms": default_ms,
            "best": best_shape, "best_ms": best_ms, "candidates": len(seen),

@@ -179,7 +179,10 @@ void ensure_tickets(pv_handle *h, size_t count) {
     dfree(h->tickets);
     h->tickets_cap = 0;
     h->tickets = dalloc<unsigned>(count, "hipMalloc(tile tickets)");
-    hip_check(hipMemset(h->tickets, 0, std::max<size_t>(count, 1) * sizeof(unsigned)),
+    // Stream-ordered: the handle's stream is non-blocking, so a memset on the null stream
+    // would not be ordered before the pair kernel that counts on these tickets.
+    hip_check(hipMemsetAsync(h->tickets, 0, std::max<size_t>(count, 1) * sizeof(unsigned),
+                             h->stream),
               "clear tile tickets");
     h->tickets_cap = count;
 }

@@ -59,12 +59,23 @@ __device__ void batch_item(const BatchLaunch &a, const Instance *ins, const Work
     for (int base = 0; base < n; base += kWaveTargets) {
         const int cnt = min(kWaveTargets, n - base);
         wave_sync();
+        bool near = false;
         if (lane < cnt) {
             const int j = off + base + lane;
-            sh[lane] = P::src(gp, a.x[j], a.y[j], a.g[j]);
+            const double xj = a.x[j], yj = a.y[j];
+            sh[lane] = P::src(gp, xj, yj, a.g[j]);
+            if constexpr (P::kNearTiles)
+                near = P::is_near(gp, xj, yj);
         }
         wave_sync();
         const bool diag = base < it.tend && it.tbegin < base + cnt;
+        if constexpr (P::kNearTiles) {
+            if (__any(near)) {
+                for (int k = 0; k < cnt; ++k)
+                    P::pair_ref_checked(gp, tg, sh[k], acc, ti == base + k);
+                continue;
+            }
+        }
         if (!diag) {
             int k = 0;
             if constexpr (P::kBatched) {

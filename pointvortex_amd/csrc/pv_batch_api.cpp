@@ -252,8 +252,7 @@ void clear_instances(pv_batch *b) {
     b->nitems = 0;
     b->items_dirty = true;
     b->steps_done = 0;
-    const unsigned zero[2] = {0u, 0u};
-    hip_check(hipMemcpy(b->counter, zero, sizeof zero, hipMemcpyHostToDevice),
+    hip_check(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), b->stream),
               "clear step count");
 }
 
@@ -361,13 +360,18 @@ pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max
             std::memset(&b->steps_h[k], 0, sizeof(pv::InstanceStep));
             b->steps_h[k].state = pv::kIdle;
         }
-        hip_check(hipMemcpy(b->steps_d, b->steps_h, max_instances * sizeof(pv::InstanceStep),
-                            hipMemcpyHostToDevice),
+        // All initialisation is ordered on the ensemble's own (non-blocking) stream.
+        hip_check(hipMemcpyAsync(b->steps_d, b->steps_h, max_instances * sizeof(pv::InstanceStep),
+                                 hipMemcpyHostToDevice, b->stream),
                   "upload instance states");
         std::vector<int> minus(n, -1);
-        hip_check(hipMemcpy(b->owner, minus.data(), n * sizeof(int), hipMemcpyHostToDevice),
+        hip_check(hipMemcpyAsync(b->owner, minus.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                                 b->stream),
                   "clear owners");
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
         b->counter = dalloc<unsigned>(2, "hipMalloc(work counter)");
+        hip_check(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), b->stream),
+                  "clear work counter");
         b->st = dalloc<pv::DevStatus>(1, "hipMalloc(status)");
         hip_check(hipHostMalloc((void **)&b->st_host, sizeof(pv::DevStatus), hipHostMallocDefault),
                   "hipHostMalloc(status)");

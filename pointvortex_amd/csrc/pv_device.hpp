@@ -201,6 +201,7 @@ struct InfF64 {
     };
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = true;
+    static constexpr bool kNearTiles = false;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, kInvTwoPi * g, 0.0};
     }
@@ -261,6 +262,7 @@ struct InfMixed {
     };
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
+    static constexpr bool kNearTiles = false;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
     }
@@ -302,9 +304,17 @@ struct InfMixed {
 // src/compute.cpp:306-329: direct term (j != i) plus the circle-theorem image of every
 // source (j == i included).  t = x_i/R, s = x_j/R, a = 1 - t.s, b = t_y s_x - t_x s_y,
 // u -= c(-a s_y + b s_x), v += c(-a s_x - b s_y), c = -G/(2 pi R (a^2+b^2)).
+// FP64 disk (src/compute.cpp:286-336).  The circle-theorem image of source j (strength G_j at
+// x_j, |x_j| < R) is a vortex of strength -G_j at the image point x_j* = R^2 x_j / |x_j|^2:
+// with t = x_i/R, s = x_j/R, a = 1 - t.s and b = t x s, the reference's image velocity
+// -G/(2 pi R (a^2 + b^2)) (-a s_y + b s_x, -a s_x - b s_y) equals the Biot-Savart velocity of
+// -G at x_j* (1 - t conj(s) = (a - i b) and x_i - x_j* = R (t - s/|s|^2)).  So each pair is two
+// plane-vortex terms with the image point computed once per staged source.  A source within
+// 2^-100 R of the centre has its image point near overflow; a tile holding one is computed
+// with the reference's a, b form instead (the tile flag is uniform, so no per-pair branch).
 struct DiskF64 {
     struct Src {
-        double x, y, g, sx, sy, gr;
+        double x, y, g, ix, iy, ig;    // g = G/(2 pi); image point and strength -g
     };
     struct Tgt {
         double x, y, tx, ty;
@@ -314,30 +324,30 @@ struct DiskF64 {
     };
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = true;
+    static constexpr bool kNearTiles = true;
+    __device__ static bool is_near(const GeoParams &p, double x, double y) {
+        return __builtin_fma(x, x, y * y) < 0x1p-200 * p.R2;
+    }
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
-        return {x, y, kInvTwoPi * g, x * p.invR, y * p.invR, kInvTwoPi * g * p.invR};
+        const double r2 = __builtin_fma(x, x, y * y);
+        const double gg = kInvTwoPi * g;
+        if (r2 < 0x1p-200 * p.R2)
+            return {x, y, gg, 0.0, 0.0, 0.0};
+        const double q = p.R2 / r2;
+        return {x, y, gg, x * q, y * q, -gg};
     }
     __device__ static Tgt tgt(const GeoParams &p, double x, double y) {
         return {x, y, x * p.invR, y * p.invR};
     }
     __device__ static void zero(Acc &a) { a.u = a.v = 0.0; }
-    __device__ static void image(const Tgt &t, const Src &s, Acc &a) {
-        const double a_ = 1.0 - __builtin_fma(t.tx, s.sx, t.ty * s.sy);
-        const double b_ = __builtin_fma(t.ty, s.sx, -(t.tx * s.sy));
-        const double den = __builtin_fma(a_, a_, b_ * b_);
-        const double k = s.gr * rcp64(den);
-        const double pu = __builtin_fma(a_, s.sy, -(b_ * s.sx));
-        const double pv = __builtin_fma(a_, s.sx, b_ * s.sy);
-        a.u = __builtin_fma(-k, pu, a.u);
-        a.v = __builtin_fma(k, pv, a.v);
-    }
-    __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, Acc &a) {
-        const double dx = t.x - s.x, dy = t.y - s.y;
-        const double r2 = __builtin_fma(dx, dx, dy * dy);
-        const double c = s.g * rcp64(r2);
+    __device__ static void point(double dx, double dy, double g, Acc &a) {
+        const double c = g * rcp64(__builtin_fma(dx, dx, dy * dy));
         a.u = __builtin_fma(-c, dy, a.u);
         a.v = __builtin_fma(c, dx, a.v);
-        image(t, s, a);
+    }
+    __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, Acc &a) {
+        point(t.x - s.x, t.y - s.y, s.g, a);
+        point(t.x - s.ix, t.y - s.iy, s.ig, a);
     }
     __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, Acc &a,
                                         bool self) {
@@ -347,34 +357,51 @@ struct DiskF64 {
         const double c = self ? 0.0 : s.g * rcp64(r2);
         a.u = __builtin_fma(-c, dy, a.u);
         a.v = __builtin_fma(c, dx, a.v);
-        image(t, s, a);
+        point(t.x - s.ix, t.y - s.iy, s.ig, a);
+    }
+    // The reference's form (src/compute.cpp:318-329), for tiles holding a near-centre source.
+    __device__ static void pair_ref_checked(const GeoParams &p, const Tgt &t, const Src &s,
+                                            Acc &a, bool self) {
+        const double dx = t.x - s.x, dy = t.y - s.y;
+        double r2 = __builtin_fma(dx, dx, dy * dy);
+        r2 = self ? 1.0 : r2;
+        const double c = self ? 0.0 : s.g * rcp64(r2);
+        a.u = __builtin_fma(-c, dy, a.u);
+        a.v = __builtin_fma(c, dx, a.v);
+        const double sx = s.x * p.invR, sy = s.y * p.invR, gr = s.g * p.invR;
+        const double a_ = 1.0 - __builtin_fma(t.tx, sx, t.ty * sy);
+        const double b_ = __builtin_fma(t.ty, sx, -(t.tx * sy));
+        const double k = gr * rcp64(__builtin_fma(a_, a_, b_ * b_));
+        const double pu = __builtin_fma(a_, sy, -(b_ * sx));
+        const double pv = __builtin_fma(a_, sx, b_ * sy);
+        a.u = __builtin_fma(-k, pu, a.u);
+        a.v = __builtin_fma(k, pv, a.v);
     }
     __device__ static void fold(Acc &) {}
     __device__ static void result(const Acc &a, double &u, double &v) {
         u = a.u;
         v = a.v;
-    }    static constexpr int kDenoms = 2;
+    }
+    static constexpr int kDenoms = 2;
     struct Tmp {
-        double dx, dy, a, b;
+        double dx, dy, ex, ey;
     };
     __device__ static void prep(const GeoParams &, const Tgt &t, const Src &s, Tmp &m,
                                 double *d) {
         m.dx = t.x - s.x;
         m.dy = t.y - s.y;
         d[0] = __builtin_fma(m.dx, m.dx, m.dy * m.dy);
-        m.a = 1.0 - __builtin_fma(t.tx, s.sx, t.ty * s.sy);
-        m.b = __builtin_fma(t.ty, s.sx, -(t.tx * s.sy));
-        d[1] = __builtin_fma(m.a, m.a, m.b * m.b);
+        m.ex = t.x - s.ix;
+        m.ey = t.y - s.iy;
+        d[1] = __builtin_fma(m.ex, m.ex, m.ey * m.ey);
     }
     __device__ static void apply(const Src &s, const Tmp &m, const double *inv, Acc &a) {
         const double c = s.g * inv[0];
         a.u = __builtin_fma(-c, m.dy, a.u);
         a.v = __builtin_fma(c, m.dx, a.v);
-        const double k = s.gr * inv[1];
-        const double pu = __builtin_fma(m.a, s.sy, -(m.b * s.sx));
-        const double pv = __builtin_fma(m.a, s.sx, m.b * s.sy);
-        a.u = __builtin_fma(-k, pu, a.u);
-        a.v = __builtin_fma(k, pv, a.v);
+        const double k = s.ig * inv[1];
+        a.u = __builtin_fma(-k, m.ey, a.u);
+        a.v = __builtin_fma(k, m.ex, a.v);
     }
 };
 
@@ -396,6 +423,7 @@ struct DiskMixed {
     };
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
+    static constexpr bool kNearTiles = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), (float)(x * p.invR), (float)(y * p.invR),
                 (float)(kInvTwoPi * g * p.invR)};
@@ -473,6 +501,7 @@ struct PerF64 {
     };
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = false;
+    static constexpr bool kNearTiles = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, p.scale * g, 0.0};
     }
@@ -559,6 +588,7 @@ struct PerMixed {
     };
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
+    static constexpr bool kNearTiles = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
     }

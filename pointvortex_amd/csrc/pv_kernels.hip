@@ -58,12 +58,31 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
     for (int base = s0; base < s1; base += TILE) {
         const int cnt = min(TILE, s1 - base);
         __syncthreads();
+        int near = 0;
         for (int k = tid; k < cnt; k += kBlock) {
             const int j = base + k;
-            sh[k] = P::src(gp, a.x[j], a.y[j], a.g[j]);
+            const double xj = a.x[j], yj = a.y[j];
+            sh[k] = P::src(gp, xj, yj, a.g[j]);
+            if constexpr (P::kNearTiles)
+                near |= P::is_near(gp, xj, yj);
         }
-        __syncthreads();
+        bool near_tile = false;
+        if constexpr (P::kNearTiles)
+            near_tile = __syncthreads_or(near) != 0;
+        else
+            __syncthreads();
         const bool diag = base < t0 + TT && t0 < base + cnt;
+        if constexpr (P::kNearTiles) {
+            if (near_tile) {
+                for (int k = 0; k < cnt; ++k) {
+                    const typename P::Src s_a = sh[k];
+#pragma unroll
+                    for (int t = 0; t < TPT; ++t)
+                        P::pair_ref_checked(gp, tg[t], s_a, acc[t], ti[t] == base + k);
+                }
+                continue;
+            }
+        }
         if (!diag) {
             int k = 0;
             if constexpr (P::kBatched) {

@@ -279,3 +279,18 @@ def test_fixed_step_failures_are_per_instance(oracle):
     assert err <= max(1e-10, 10 * amplified), (err, amplified)
     with pytest.raises(Exception, match="outside the disk"):
         oracle.integrator(4).rk4_step(spec_disk, ox.copy(), oy.copy(), wg, dt)
+
+
+def test_ensemble_disk_near_centre_source(oracle):
+    """The ensemble kernel's per-wave tiles take the reference's image form when a source sits
+    within 2^-100 R of the centre."""
+    pops = [wl.disk(150, 61, wall_fraction=0.0), wl.disk(90, 62, wall_fraction=0.0)]
+    pops[0].x[17], pops[0].y[17] = 1e-155, 0.0
+    pops[1].x[80], pops[1].y[80] = 0.0, 0.0
+    cfgs = [InstanceConfig("disk"), InstanceConfig("disk")]
+    e = Ensemble(sum(p.n for p in pops), 4)
+    e.load(cfgs, [p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
+    us, vs = e.evaluate()
+    for c, p, u, v in zip(cfgs, pops, us, vs):
+        ur, vr = oracle.evaluate(ospec(c), p.x, p.y, p.circulation)
+        assert normwise(u, v, ur, vr) <= fp64_gate("disk", p.x, p.y, p.circulation, ur, vr)

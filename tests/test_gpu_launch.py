@@ -88,7 +88,7 @@ def test_table_from_environment(tmp_path):
 @pytest.mark.parametrize("geometry,precision", [("infinite", "fp64"), ("disk", "mixed")])
 def test_autotuner_choice_keeps_parity(oracle, geometry, precision):
     res = autotune.tune_one(geometry, precision, 2048, log=lambda *a: None)
-    assert res["best_ms"] <= res["default_ms"]
+    assert res["best_ms"] <= res["default_ms"] * (1.0 + autotune.TIE)
     tpt, splits, tile = res["best"]
     pop = autotune.population(geometry, 2048)
     k = HipKernel(SimParams(boundaryCondition=geometry, precision=precision,

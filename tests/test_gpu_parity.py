@@ -382,3 +382,62 @@ def test_dipole_candidates_parity(oracle, geometry):
     got = sorted(zip(d2.tolist(), f.tolist(), s.tolist()))
     want = sorted(zip(d2o.tolist(), fo.tolist(), so.tolist()))
     assert got == want
+
+
+@pytest.mark.parametrize("precision,gate", [("fp64", None), ("mixed", 1e-6)])
+def test_disk_near_centre_source_in_a_population(oracle, precision, gate):
+    """A source within 2^-100 R of the centre puts its tile on the reference's a, b form for
+    the image (its image point would overflow); the other tiles use the image-point form."""
+    pop = wl.disk(1200, 41)
+    x, y = pop.x.copy(), pop.y.copy()
+    x[700], y[700] = 3e-160, -1e-161      # one near-centre source (two would nearly coincide)
+    k = kernel_for("disk", precision)
+    vel = k.evaluate(VortexSystem(x, y, pop.circulation))
+    ur, vr = oracle.evaluate(spec("disk"), x, y, pop.circulation)
+    if gate is None:
+        gate = fp64_gate("disk", x, y, pop.circulation, ur, vr)
+    assert normwise(vel.x, vel.y, ur, vr) <= gate
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_fp64_parity_at_n65536_on_target_subsets(oracle, geometry):
+    """The benchmark's size: the full N=65536 evaluation on the device against the oracle's
+    sums for 768 targets (the first 256, 256 from the middle and the last 256, which for the
+    disk are the wall shell at r in [0.995, 0.9999]) over all sources.  For the disk the bar
+    is max(1e-12, 2x the oracle's own distance from an extended-precision sum on the same
+    targets)."""
+    n = 65536
+    if geometry == "infinite":
+        pop = wl.infinite(n, 0)
+    elif geometry == "periodic":
+        pop = wl.periodic(n, 1)
+    else:
+        pop = wl.disk(n, 2)
+    kw = {"boxLengthX": TWO_PI, "boxLengthY": TWO_PI}
+    k = kernel_for(geometry, **kw)
+    vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+    s = spec_for(geometry, boxLengthX=TWO_PI)
+    for b, e in [(0, 256), (n // 2, n // 2 + 256), (n - 256, n)]:
+        ur, vr = oracle.evaluate_range(s, pop.x, pop.y, pop.circulation, b, e)
+        ur, vr = ur[b:e], vr[b:e]
+        gate = 1e-12
+        if geometry == "disk":
+            from tests.conftest import disk_longdouble_targets
+            ue, ve = disk_longdouble_targets(pop.x, pop.y, pop.circulation, np.arange(b, e))
+            gate = max(gate, 2.0 * normwise(ur, vr, ue, ve))
+        assert normwise(vel.x[b:e], vel.y[b:e], ur, vr) <= gate, (b, e)
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_mixed_parity_at_n65536_on_target_subsets(oracle, geometry):
+    """Mixed precision at the benchmark's size against the FP64 oracle, on the same target
+    subsets (normwise 1e-6)."""
+    n = 65536
+    pop = {"infinite": lambda: wl.infinite(n, 0), "periodic": lambda: wl.periodic(n, 1),
+           "disk": lambda: wl.disk(n, 2)}[geometry]()
+    k = kernel_for(geometry, "mixed", boxLengthX=TWO_PI, boxLengthY=TWO_PI)
+    vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+    s = spec_for(geometry, boxLengthX=TWO_PI)
+    for b, e in [(0, 256), (n - 256, n)]:
+        ur, vr = oracle.evaluate_range(s, pop.x, pop.y, pop.circulation, b, e)
+        assert normwise(vel.x[b:e], vel.y[b:e], ur[b:e], vr[b:e]) <= 1e-6, (b, e)
