@@ -31,8 +31,23 @@ CASES = (
     + [(g + "_dopri", g, ORDINARY, .003, "dopri5") for g in ("infinite", "periodic", "disk")]
 )
 
+# Dipole events through the reference's own host code (src/dipole.cpp): a +-1 pair closer than
+# the removal distance is removed and, for the periodic box and the disk, reinjected with the
+# driver's seeded generator (src/params.h:64-69).
+DIPOLE = ORDINARY + [(.05, .05, 1), (.054, .05, -1)]
+DIPOLE_CASES = [
+    ("periodic_dipole_independent", "periodic", DIPOLE, .01, "dopri5",
+     "dipoleRemoval true\ndipoleRemovalDistance 0.01\ndipoleReinjection independent\n"
+     "randomSeed 11\n"),
+    ("disk_dipole_paired", "disk", DIPOLE, .01, "rk4",
+     "dipoleRemoval true\ndipoleRemovalDistance 0.01\ndipoleReinjection paired\n"
+     "randomSeed 5\n"),
+    ("infinite_dipole_removed", "infinite", DIPOLE, .01, "rk4",
+     "dipoleRemoval true\ndipoleRemovalDistance 0.01\n"),
+]
 
-def _run(exe, directory, geometry, rows, end, integrator):
+
+def _run(exe, directory, geometry, rows, end, integrator, extra=""):
     directory.mkdir()
     initial = directory / "initial.dat"
     initial.write_text("".join(f"{x:.17g} {y:.17g} {g:.17g}\n" for x, y, g in rows))
@@ -41,7 +56,7 @@ def _run(exe, directory, geometry, rows, end, integrator):
         f"initialConditionFile {initial}\nboundaryCondition {geometry}\n"
         f"endTime {end}\nintegrator {integrator}\ntimeStep 0.001\noutputTime 0.002\n"
         "diagnosticsTime 0.001\ncheckpointTime 0.003\nnumThreads 2\n"
-        f"runDirectory {directory}\n")
+        f"runDirectory {directory}\n{extra}")
     r = subprocess.run([str(exe), str(params)], text=True, capture_output=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     return r.stdout
@@ -98,3 +113,21 @@ def test_driver_reports_reference_errors(executables, tmp_path):
         assert r.returncode == 1
         msgs.append(r.stderr.strip())
     assert msgs[0] == msgs[1] == "error: coincident periodic vortices"
+
+
+@pytest.mark.parametrize("name,geometry,rows,end,integrator,extra", DIPOLE_CASES,
+                         ids=[c[0] for c in DIPOLE_CASES])
+def test_driver_dipole_events_match_cpu_reference(executables, tmp_path, name, geometry, rows,
+                                                  end, integrator, extra):
+    cpu, hip = executables
+    _run(cpu, tmp_path / "cpu", geometry, rows, end, integrator, extra)
+    out = _run(hip, tmp_path / "hip", geometry, rows, end, integrator, extra)
+    assert "backend=HIP" in out
+    for fname in ("trajectory.csv", "diagnostics.csv"):
+        _compare(tmp_path / "cpu" / fname, tmp_path / "hip" / fname)
+    # the event happened (diagnostics.csv counts removed and reinjected pairs)
+    with open(tmp_path / "hip" / "diagnostics.csv") as f:
+        last = list(csv.DictReader(f))[-1]
+    assert int(last["removed_pairs"]) >= 1, name
+    assert int(last["reinjected_pairs"]) == (0 if geometry == "infinite" else
+                                             int(last["removed_pairs"])), name
