@@ -204,6 +204,44 @@ def ensemble_bench(pv, steps, count=4096):
     return out
 
 
+def c2_slice(pv, accepted=40, budget_s=20.0):
+    """The periodic configuration (L=2 pi, N=16384, +-1, seed 1) as a slice: DOPRI5 at rtol
+    1e-9, atol 1e-12 from dt=1e-3 for `accepted` accepted steps (or `budget_s`), with the
+    device dipole-candidate search (removal distance 1e-3 L) after every accepted step, as
+    the driver's event check runs it.  Pairs are metered as N(N-1) per evaluation: six per
+    attempt after the first (FSAL), seven for the first."""
+    from pointvortex_amd import workloads as wl
+
+    pop = wl.config2()
+    n = pop.n
+    k = pv.HipKernel(pv.SimParams(boundaryCondition="periodic", boxLengthX=2 * np.pi,
+                                  boxLengthY=2 * np.pi))
+    k.uploadDeviceState(pv.VortexSystem(pop.x, pop.y, pop.circulation))
+    dt, t, steps, attempts, cand = 1e-3, 0.0, 0, 0, 0
+    search_s = 0.0
+    t0 = time.perf_counter()
+    while steps < accepted and time.perf_counter() - t0 < budget_s:
+        r = k.deviceDopri5Step(dt, 1e-12, 1e-9, 1e-12, 0.1)
+        attempts += 1 + r.rejectedSteps
+        steps += 1
+        t += r.acceptedTimeStep
+        dt = r.suggestedTimeStep
+        s0 = time.perf_counter()
+        cand += len(k.dipoleCandidates(1e-3 * 2 * np.pi)[0])
+        search_s += time.perf_counter() - s0
+    wall = time.perf_counter() - t0
+    evals = 6 * attempts + 1
+    out = {"n": n, "accepted_steps": steps, "attempts": attempts, "t_reached": t,
+           "wall_s": wall, "attempts_per_s": attempts / wall,
+           "pairs_per_s": evals * n * (n - 1) / wall,
+           "dipole_search_s_per_step": search_s / max(1, steps),
+           "dipole_candidates_seen": cand}
+    log(f"[bench] c2 slice: {steps} accepted steps, {attempts} attempts in {wall:.2f} s "
+        f"({out['pairs_per_s']:.3e} pairs/s), t={t:.3e}")
+    k.close()
+    return out
+
+
 def serving_bench(pv, count=128, seed=5):
     """Continuous-batching request loop, scaled down so that it runs in about a minute:
     N log-uniform in [64, 2048], horizon U[1e-4, 1e-3], rtol log-uniform in [1e-10, 1e-6],
@@ -272,7 +310,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the per-geometry table, the ensemble and the serving loop")
     ap.add_argument("--ensemble-steps", type=int, default=500)
-    ap.add_argument("--only", choices=["fp64", "geometries", "copy", "ensemble", "serving"],
+    ap.add_argument("--only", choices=["fp64", "geometries", "copy", "ensemble", "serving", "c2"],
                     default=None,
                     help="profiling runs: only the FP64 RK4 run, only the per-geometry table, "
                          "or only the 256 MiB copy-kernel calibration")
@@ -299,6 +337,9 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps(serving_bench(pv)), flush=True)
+        return
+    if args.only == "c2":
+        print(json.dumps(c2_slice(pv)), flush=True)
         return
     pop = wl.disk(n, 2)
     pairs_per_step = 4 * n * (n - 1)
@@ -380,6 +421,7 @@ def main():
     }
     if not args.no_extra:
         result["per_geometry_n65536"] = per_geometry(pv, n)
+        result["periodic_n16384_dopri5_slice"] = c2_slice(pv)
         result["ensemble_4096"] = ensemble_bench(pv, args.ensemble_steps)
         result["serving"] = serving_bench(pv)
     if not args.no_cpu:

@@ -405,17 +405,21 @@ struct DiskF64 {
     }
 };
 
-// Mixed disk.  With d = x_i - x_j formed in FP64 and w = 1 - |t|^2 formed in FP64 per target,
-// a = w + (t/R).d and b = (t/R) x d are the same a, b as above (a = 1 - t.(t - d/R)) but
-// without the cancellation of 1 - t.s near the wall, so they can be evaluated in FP32.
+// Mixed disk.  The direct term as in the infinite plane (FP64 difference d = x_i - x_j, FP32
+// after).  The image term is the plane-vortex term of -G_j at the image point x_j* (see
+// DiskF64), with x_i - x_j* = d - delta_j, where delta_j = x_j* - x_j = x_j (R^2/|x_j|^2 - 1) is
+// formed in FP64 once per source and stored in FP32.  The subtraction d - delta_j is safe in
+// FP32: x_i is inside the wall and x_j* outside it, so |d - delta_j| is never much smaller than
+// |d| + |delta_j| (within a factor ~3), and its rounding stays a few FP32 ulps.  A source within
+// 1e-30 R of the centre gets a far, finite offset: its image term then evaluates to 0 (its true
+// size is below 1e-30 of the source's own scale) instead of overflowing.
 struct DiskMixed {
     struct Src {
         double x, y;
-        float g, sx, sy, gr;
+        float g, ix, iy, pad;      // g = G/(2 pi); (ix, iy) = delta
     };
     struct Tgt {
         double x, y;
-        float trx, try_, w, pad;
     };
     struct Acc {
         float u32, v32;
@@ -425,48 +429,44 @@ struct DiskMixed {
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
-        return {x, y, (float)(kInvTwoPi * g), (float)(x * p.invR), (float)(y * p.invR),
-                (float)(kInvTwoPi * g * p.invR)};
+        const double r2 = __builtin_fma(x, x, y * y);
+        float ix = 1e30f, iy = 0.f;
+        if (r2 >= 1e-60 * p.R2) {
+            const double q = p.R2 / r2 - 1.0;
+            ix = (float)fmin(fmax(x * q, -1e30), 1e30);
+            iy = (float)fmin(fmax(y * q, -1e30), 1e30);
+        }
+        return {x, y, (float)(kInvTwoPi * g), ix, iy, 0.f};
     }
-    __device__ static Tgt tgt(const GeoParams &p, double x, double y) {
-        const double tx = x * p.invR, ty = y * p.invR;
-        const double w = 1.0 - __builtin_fma(tx, tx, ty * ty);
-        return {x, y, (float)(tx * p.invR), (float)(ty * p.invR), (float)w, 0.f};
-    }
+    __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
     __device__ static void zero(Acc &a) {
         a.u32 = a.v32 = 0.f;
         a.u.zero();
         a.v.zero();
     }
-    __device__ static void body(const Tgt &t, const Src &s, Acc &a, float cdir) {
-        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
-        const float r2 = __builtin_fmaf(dx, dx, dy * dy);
-        const float c = cdir * rcp32(r2);
-        a.u32 = __builtin_fmaf(-c, dy, a.u32);
-        a.v32 = __builtin_fmaf(c, dx, a.v32);
-        const float a_ = __builtin_fmaf(t.trx, dx, __builtin_fmaf(t.try_, dy, t.w));
-        const float b_ = __builtin_fmaf(t.trx, dy, -(t.try_ * dx));
-        const float den = __builtin_fmaf(a_, a_, b_ * b_);
-        const float k = s.gr * rcp32(den);
-        const float pu = __builtin_fmaf(a_, s.sy, -(b_ * s.sx));
-        const float pv = __builtin_fmaf(a_, s.sx, b_ * s.sy);
-        a.u32 = __builtin_fmaf(-k, pu, a.u32);
-        a.v32 = __builtin_fmaf(k, pv, a.v32);
+    __device__ static void image(const Src &s, float dx, float dy, Acc &a) {
+        const float ex = dx - s.ix, ey = dy - s.iy;
+        const float k = s.g * rcp32(__builtin_fmaf(ex, ex, ey * ey));
+        a.u32 = __builtin_fmaf(k, ey, a.u32);
+        a.v32 = __builtin_fmaf(-k, ex, a.v32);
     }
     __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, Acc &a) {
-        body(t, s, a, s.g);
+        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+        const float c = s.g * rcp32(__builtin_fmaf(dx, dx, dy * dy));
+        a.u32 = __builtin_fmaf(-c, dy, a.u32);
+        a.v32 = __builtin_fmaf(c, dx, a.v32);
+        image(s, dx, dy, a);
     }
     __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, Acc &a,
                                         bool self) {
-        // The self pair has no direct term; its own image has d = 0, so a = w and b = 0.
-        if (self) {
-            const float a_ = t.w;
-            const float k = s.gr * rcp32(a_ * a_);
-            a.u32 = __builtin_fmaf(-k, a_ * s.sy, a.u32);
-            a.v32 = __builtin_fmaf(k, a_ * s.sx, a.v32);
-        } else {
-            body(t, s, a, s.g);
-        }
+        // The self pair has no direct term; its own image (d = 0) stays.
+        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+        float r2 = __builtin_fmaf(dx, dx, dy * dy);
+        r2 = self ? 1.f : r2;
+        const float c = self ? 0.f : s.g * rcp32(r2);
+        a.u32 = __builtin_fmaf(-c, dy, a.u32);
+        a.v32 = __builtin_fmaf(c, dx, a.v32);
+        image(s, dx, dy, a);
     }
     __device__ static void fold(Acc &a) {
         a.u.add((double)a.u32);
