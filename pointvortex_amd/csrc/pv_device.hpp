@@ -155,6 +155,41 @@ __device__ __forceinline__ double cos_f64(double h2) {
     p = __builtin_fma(p, h2, 0.5);
     return __builtin_fma(-p, h2, 1.0);
 }
+// sinh(h) and sin(h) together: with z = h^2, sinh(h) = h P(z) and sin(h) = h P(-z), where
+// P(z) = sum_k z^k/(2k+1)!.  Splitting P into even and odd powers, P(+-z) = A(z^2) +- z B(z^2),
+// evaluates both in one pass (A and B are Horner polynomials in z^2 with the same 1/(2k+1)!
+// coefficients as sinh_f64/sin_f64, so the truncation is the same).  On |h| <= pi/2 the sin
+// branch A - z B stays above 2/pi: no cancellation.
+__device__ __forceinline__ void sinh_sin_f64(double h, double h2, double &sh, double &sn) {
+    const double w = h2 * h2;
+    double a = 1.0 / 51090942171709440000.0;              // 1/21!
+    a = __builtin_fma(a, w, 1.0 / 355687428096000.0);      // 1/17!
+    a = __builtin_fma(a, w, 1.0 / 6227020800.0);           // 1/13!
+    a = __builtin_fma(a, w, 1.0 / 362880.0);               // 1/9!
+    a = __builtin_fma(a, w, 1.0 / 120.0);                  // 1/5!
+    a = __builtin_fma(a, w, 1.0);                          // 1/1!
+    double b = 1.0 / 121645100408832000.0;                 // 1/19!
+    b = __builtin_fma(b, w, 1.0 / 1307674368000.0);        // 1/15!
+    b = __builtin_fma(b, w, 1.0 / 39916800.0);             // 1/11!
+    b = __builtin_fma(b, w, 1.0 / 5040.0);                 // 1/7!
+    b = __builtin_fma(b, w, 1.0 / 6.0);                    // 1/3!
+    const double zb = h2 * b;
+    sh = h * (a + zb);
+    sn = h * (a - zb);
+}
+__device__ __forceinline__ void sinh_sin_f32(float h, float h2, float &sh, float &sn) {
+    const float w = h2 * h2;
+    float a = 1.0f / 6227020800.0f;                        // 1/13!
+    a = __builtin_fmaf(a, w, 1.0f / 362880.0f);            // 1/9!
+    a = __builtin_fmaf(a, w, 1.0f / 120.0f);               // 1/5!
+    a = __builtin_fmaf(a, w, 1.0f);
+    float b = 1.0f / 39916800.0f;                          // 1/11!
+    b = __builtin_fmaf(b, w, 1.0f / 5040.0f);              // 1/7!
+    b = __builtin_fmaf(b, w, 1.0f / 6.0f);                 // 1/3!
+    const float zb = h2 * b;
+    sh = h * (a + zb);
+    sn = h * (a - zb);
+}
 __device__ __forceinline__ float sinh_f32(float h, float h2) {
     float p = 1.0f / 6227020800.0f;
     p = __builtin_fmaf(p, h2, 1.0f / 39916800.0f);
@@ -514,8 +549,9 @@ struct PerF64 {
         const double Y = p.wave * remainder_fast(t.y - s.y, p.L, p.invL);
         const double hx = 0.5 * X, hy = 0.5 * Y;
         const double hx2 = hx * hx, hy2 = hy * hy;
-        const double shx = sinh_f64(hx, hx2), shy = sinh_f64(hy, hy2);
-        const double snx = sin_f64(hx, hx2), sny = sin_f64(hy, hy2);
+        double shx, shy, snx, sny;
+        sinh_sin_f64(hx, hx2, shx, snx);
+        sinh_sin_f64(hy, hy2, shy, sny);
         const double csx = cos_f64(hx2), csy = cos_f64(hy2);
         const double snx2 = snx * snx, sny2 = sny * sny;
         const double shx2 = shx * shx, shy2 = shy * shy;
@@ -604,8 +640,9 @@ struct PerMixed {
         const float Y = (float)(p.wave * remainder_fast(t.y - s.y, p.L, p.invL));
         const float hx = 0.5f * X, hy = 0.5f * Y;
         const float hx2 = hx * hx, hy2 = hy * hy;
-        const float shx = sinh_f32(hx, hx2), shy = sinh_f32(hy, hy2);
-        const float snx = sin_f32(hx, hx2), sny = sin_f32(hy, hy2);
+        float shx, shy, snx, sny;
+        sinh_sin_f32(hx, hx2, shx, snx);
+        sinh_sin_f32(hy, hy2, shy, sny);
         const float csx = cos_f32(hx2), csy = cos_f32(hy2);
         const float snx2 = snx * snx, sny2 = sny * sny;
         const float shx2 = shx * shx, shy2 = shy * shy;
