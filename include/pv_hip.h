@@ -163,13 +163,15 @@ pv_status pv_synchronize(pv_handle *h);
 
 /* Launch shapes of the pair kernels (no reference counterpart; the CUDA backend used one
  * fixed shape, src/backend_cuda.cu:402-416).  A shape is targets per thread (1, 2 or 4; 0 =
- * automatic), source splits (0 = automatic) and the LDS source tile (128 or 256; 0 = the
- * precision's default).  Precedence: pv_set_launch / pv_set_launch_shape, then the first
- * matching rule of the handle's launch table, then a built-in heuristic.  A new handle's
- * table is read from the file named by $PV_HIP_LAUNCH_TABLE, else from launch_table.txt
- * beside libpvhip.so when present (lines "geometry precision n_max tpt splits tile", as
- * written by the autotuner, pointvortex_amd/autotune.py).  pv_resolve_launch reports the
- * shape an evaluation of `targets` targets against n sources would use. */
+ * automatic), source splits (0 = automatic), the LDS source tile (128 or 256; 0 = the
+ * precision's default) and whether whole-population FP64 evaluations use the symmetric path
+ * (each unordered pair computed once; 0 or 1).  Precedence: pv_set_launch /
+ * pv_set_launch_shape (symmetric -1 = not forced), then the first matching rule of the handle's
+ * launch table, then a built-in heuristic (split path).  A new handle's table is read from the
+ * file named by $PV_HIP_LAUNCH_TABLE, else from launch_table.txt beside libpvhip.so when present
+ * (lines "geometry precision n_max tpt splits tile [symmetric]", as written by the autotuner,
+ * pointvortex_amd/autotune.py).  pv_resolve_launch reports the shape an evaluation of
+ * `targets` targets against n sources would use. */
 typedef struct pv_launch_rule {
     int geometry;            /* pv_geometry */
     int precision;           /* pv_precision */
@@ -177,14 +179,15 @@ typedef struct pv_launch_rule {
     int targets_per_thread;
     int source_splits;
     int tile;
+    int symmetric;
 } pv_launch_rule;
 
 pv_status pv_set_launch(pv_handle *h, int targets_per_thread, int source_splits);
 pv_status pv_set_launch_shape(pv_handle *h, int targets_per_thread, int source_splits,
-                              int tile);
+                              int tile, int symmetric);
 pv_status pv_set_launch_table(pv_handle *h, const pv_launch_rule *rules, size_t count);
 pv_status pv_resolve_launch(pv_handle *h, size_t targets, size_t n, int *targets_per_thread,
-                            int *source_splits, int *tile);
+                            int *source_splits, int *tile, int *symmetric);
 
 /* Kernel timing for benchmarks (no reference counterpart): between begin and end every
  * velocity-kernel launch of the handle is bracketed by HIP events recorded on the handle's own

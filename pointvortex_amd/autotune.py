@@ -1,7 +1,8 @@
 """Autotuner of the pair-kernel launch shape per (geometry, precision, N-bucket).
 
-A shape is (targets per thread, source splits, LDS source tile); its blocks per CU follow from
-the split count.  For every geometry, precision and N-bucket the tuner uploads a seeded
+A shape is (targets per thread, source splits, LDS source tile, symmetric); its blocks per CU
+follow from the split count.  The symmetric path (FP64 only) computes each unordered pair once
+over pairs of tiles; its candidates vary targets per thread and the tile.  For every geometry, precision and N-bucket the tuner uploads a seeded
 population at the bucket's upper N, evaluates it once with the built-in shape (the
 reference result), then for every distinct candidate shape:
 
@@ -15,7 +16,8 @@ Among the shapes that passed the gate, the one with the fewest source splits wit
 `TIE` (2 %) of the fastest wins: each extra split writes and re-reads one partial sum per
 target, so a shape that is not measurably faster should not buy its HBM traffic.  The table
 is text, one rule per line:
-``geometry precision n_max targets_per_thread source_splits tile`` ('#' comments), the format
+``geometry precision n_max targets_per_thread source_splits tile symmetric`` ('#' comments;
+the last column may be left out and means 0), the format
 libpvhip.so reads from $PV_HIP_LAUNCH_TABLE or from launch_table.txt beside the library
 (pointvortex_amd.build installs pointvortex_amd/tuning/launch_table.txt there).
 
@@ -109,8 +111,12 @@ def tune_one(geometry, precision, n, log=_log):
     timed = {default_shape: default_ms}
     seen = {default_shape}
     rejected = 0
-    for tpt, tile, splits in itertools.product(TPT, TILES, SPLITS):
-        k.set_launch(tpt, splits, tile)
+    candidates = [(tpt, tile, splits, 0) for tpt, tile, splits in
+                  itertools.product(TPT, TILES, SPLITS)]
+    if precision == "fp64":
+        candidates += [(tpt, tile, 0, 1) for tpt, tile in itertools.product(TPT, TILES)]
+    for tpt, tile, splits, sym in candidates:
+        k.set_launch(tpt, splits, tile, sym)
         shape = k.resolved_launch(n, n)
         if shape in seen:
             continue
@@ -128,7 +134,8 @@ def tune_one(geometry, precision, n, log=_log):
     k.close()
     fastest = min(timed.values())
     near = [s for s, ms in timed.items() if ms <= fastest * (1.0 + TIE)]
-    best_shape = min(near, key=lambda s: (s[1], timed[s]))
+    # fewest partial sums: a split shape writes splits x 2N, a symmetric one (N/(256 tpt)) x 2N
+    best_shape = min(near, key=lambda s: (partial_slots(s, n), timed[s]))
     best_ms = timed[best_shape]
     res = {"geometry": geometry, "precision": precision, "n": n,
            "default": default_shape, "default_ms": default_ms,
@@ -138,6 +145,11 @@ def tune_one(geometry, precision, n, log=_log):
         f"{default_ms:.4f} ms -> {best_shape} {best_ms:.4f} ms "
         f"({len(seen)} shapes, {rejected} failed the gate)")
     return res
+
+
+def partial_slots(shape, n):
+    tpt, splits, _, sym = shape
+    return -(-n // (256 * tpt)) if sym else (splits if splits > 1 else 0)
 
 
 def tune(geometries=GEOMETRIES, precisions=PRECISIONS, buckets=BUCKETS, log=_log):
@@ -151,7 +163,7 @@ def tune(geometries=GEOMETRIES, precisions=PRECISIONS, buckets=BUCKETS, log=_log
 def table_lines(results):
     """Rules in table order; the largest bucket of each (geometry, precision) also covers
     every larger N."""
-    lines = ["# geometry precision n_max targets_per_thread source_splits tile"]
+    lines = ["# geometry precision n_max targets_per_thread source_splits tile symmetric"]
     groups = {}
     for r in results:
         groups.setdefault((r["geometry"], r["precision"]), []).append(r)
@@ -159,26 +171,29 @@ def table_lines(results):
         rs = sorted(rs, key=lambda r: r["n"])
         for i, r in enumerate(rs):
             n_max = SIZE_MAX if i == len(rs) - 1 else r["n"]
-            tpt, splits, tile = r["best"]
-            lines.append(f"{geometry} {precision} {n_max} {tpt} {splits} {tile}  "
+            tpt, splits, tile, sym = r["best"]
+            lines.append(f"{geometry} {precision} {n_max} {tpt} {splits} {tile} {sym}  "
                          f"# N={r['n']}: {r['default_ms']:.4f} ms built-in {r['default']}, "
                          f"{r['best_ms']:.4f} ms tuned")
     return lines
 
 
 def read_table(path):
-    """Rules of a table file as tuples (geometry, precision, n_max, tpt, splits, tile)."""
+    """Rules of a table file as tuples (geometry, precision, n_max, tpt, splits, tile,
+    symmetric)."""
     rules = []
     for line in Path(path).read_text().splitlines():
         line = line.split("#", 1)[0].split()
         if not line:
             continue
-        if len(line) != 6:
+        if len(line) == 6:
+            line = line + ["0"]
+        if len(line) != 7 or line[6] not in ("0", "1"):
             raise ValueError(f"malformed launch rule: {' '.join(line)}")
-        g, p, n, t, s, tl = line
+        g, p, n, t, s, tl, sy = line
         if g not in GEOMETRIES or p not in PRECISIONS:
             raise ValueError(f"unknown geometry or precision in launch rule: {' '.join(line)}")
-        rules.append((g, p, int(n), int(t), int(s), int(tl)))
+        rules.append((g, p, int(n), int(t), int(s), int(tl), int(sy)))
     return rules
 
 

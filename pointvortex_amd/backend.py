@@ -97,6 +97,7 @@ class PvLaunchRule(ctypes.Structure):
         ("targets_per_thread", ctypes.c_int),
         ("source_splits", ctypes.c_int),
         ("tile", ctypes.c_int),
+        ("symmetric", ctypes.c_int),
     ]
 
 
@@ -147,9 +148,11 @@ _SIGNATURES = {
     "pv_evaluate_resident": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pv_synchronize": (ctypes.c_int, [_vp]),
     "pv_set_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
-    "pv_set_launch_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pv_set_launch_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int]),
     "pv_set_launch_table": (ctypes.c_int, [_vp, ctypes.POINTER(PvLaunchRule), ctypes.c_size_t]),
     "pv_resolve_launch": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]),
@@ -310,26 +313,28 @@ class HipKernel:
         _check(_lib.pv_set_precision(self._h, PRECISIONS[precision]))
         self.params.precision = precision
 
-    def set_launch(self, targets_per_thread=0, source_splits=0, tile=0):
-        """Explicit launch shape of the pair kernels (0 = automatic / default)."""
+    def set_launch(self, targets_per_thread=0, source_splits=0, tile=0, symmetric=-1):
+        """Explicit launch shape of the pair kernels (0 = automatic / default; symmetric -1 =
+        not forced, 0 = split path, 1 = symmetric path for whole-population FP64 evaluations)."""
         _check(_lib.pv_set_launch_shape(self._h, int(targets_per_thread), int(source_splits),
-                                        int(tile)))
+                                        int(tile), int(symmetric)))
 
     def set_launch_table(self, rules):
         """rules: iterable of (geometry, precision, n_max, targets_per_thread, source_splits,
-        tile); the first rule matching the handle's geometry, precision and N applies."""
-        rules = list(rules)
+        tile[, symmetric]); the first rule matching the handle's geometry, precision and N
+        applies."""
+        rules = [tuple(r) + (0,) * (7 - len(r)) for r in rules]
         arr = (PvLaunchRule * max(1, len(rules)))(
-            *[PvLaunchRule(GEOMETRIES[g], PRECISIONS[p], int(n), int(t), int(s), int(tl))
-              for g, p, n, t, s, tl in rules])
+            *[PvLaunchRule(GEOMETRIES[g], PRECISIONS[p], int(n), int(t), int(s), int(tl),
+                           int(sy)) for g, p, n, t, s, tl, sy in rules])
         _check(_lib.pv_set_launch_table(self._h, arr, len(rules)))
 
     def resolved_launch(self, targets, n):
-        """(targets_per_thread, source_splits, tile) an evaluation would use."""
-        t, s, tl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        """(targets_per_thread, source_splits, tile, symmetric) an evaluation would use."""
+        t, s, tl, sy = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(_lib.pv_resolve_launch(self._h, int(targets), int(n), ctypes.byref(t),
-                                      ctypes.byref(s), ctypes.byref(tl)))
-        return t.value, s.value, tl.value
+                                      ctypes.byref(s), ctypes.byref(tl), ctypes.byref(sy)))
+        return t.value, s.value, tl.value, sy.value
 
     # -- pure evaluation (src/compute.h:21-27)
     def evaluateRange(self, x, y, circulation, begin, end, velocity: Optional[VectorField] = None):

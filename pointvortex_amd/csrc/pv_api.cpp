@@ -82,7 +82,7 @@ struct pv_handle {
     pv::DevStatus *st = nullptr;
     pv::DevStatus *st_host = nullptr;
 
-    int tpt_override = 0, split_override = 0, tile_override = 0;
+    int tpt_override = 0, split_override = 0, tile_override = 0, sym_override = -1;
     std::vector<pv_launch_rule> table;   // tuned shapes, first matching rule wins
     bool profiling = false;
     std::vector<hipEvent_t> events;   // pool, two per profiled launch
@@ -219,7 +219,11 @@ void begin_sequence(pv_handle *h) {
 struct Shape {
     int tpt, tile;
     size_t tiles, nsplit, split_len;
+    bool sym;      // symmetric path (whole-population FP64 evaluations only)
 };
+
+// Partial sums of the symmetric path are nt x 2N doubles; beyond this the split path runs.
+constexpr size_t kSymPartialLimit = (size_t)8 << 30;
 
 // Launch shape of an evaluation of `targets` targets against n sources.  Precedence: an
 // explicit pv_set_launch_shape, then the first matching rule of the handle's launch table,
@@ -227,10 +231,11 @@ struct Shape {
 // for about 4 blocks per CU).
 Shape resolve_shape(const pv_handle *h, size_t targets, size_t n) {
     const int prec = h->cfg.precision;
-    int tpt = 0, splits = 0, tile = 0;
+    int tpt = 0, splits = 0, tile = 0, sym = 0;
     for (const pv_launch_rule &r : h->table) {
         if (r.geometry == h->cfg.geometry && r.precision == prec && n <= r.n_max) {
             tpt = r.targets_per_thread, splits = r.source_splits, tile = r.tile;
+            sym = r.symmetric;
             break;
         }
     }
@@ -240,6 +245,8 @@ Shape resolve_shape(const pv_handle *h, size_t targets, size_t n) {
         splits = h->split_override;
     if (h->tile_override > 0)
         tile = h->tile_override;
+    if (h->sym_override >= 0)
+        sym = h->sym_override;
     if (tpt <= 0)
         tpt = targets >= 32768 ? 2 : 1;
     tpt = tpt >= 4 ? 4 : (tpt >= 2 ? 2 : 1);
@@ -260,6 +267,9 @@ Shape resolve_shape(const pv_handle *h, size_t targets, size_t n) {
     split_len = std::max<size_t>(tile, (split_len + tile - 1) / tile * tile);
     s.split_len = split_len;
     s.nsplit = std::max<size_t>(1, (n + split_len - 1) / split_len);
+    const size_t tt = (size_t)pv::kBlock * tpt, nt = (n + tt - 1) / tt;
+    s.sym = sym == 1 && pv::sym_supported(h->cfg.geometry, prec) && targets == n && n > 0 &&
+            nt * 2 * n * sizeof(double) <= kSymPartialLimit;
     return s;
 }
 
@@ -287,6 +297,16 @@ std::vector<pv_launch_rule> read_launch_table(const std::string &path) {
         if (!(ss >> prec >> n_max >> r.targets_per_thread >> r.source_splits >> r.tile))
             raise(PV_ERR_INVALID_ARGUMENT,
                   "launch table " + path + ": line " + std::to_string(lineno) + " is malformed");
+        std::string extra;
+        if (ss >> extra) {
+            if (extra != "0" && extra != "1")
+                raise(PV_ERR_INVALID_ARGUMENT, "launch table " + path + ": line " +
+                                                   std::to_string(lineno) + " is malformed");
+            r.symmetric = extra == "1";
+            if (ss >> extra)
+                raise(PV_ERR_INVALID_ARGUMENT, "launch table " + path + ": line " +
+                                                   std::to_string(lineno) + " is malformed");
+        }
         r.geometry = geo == "infinite" ? PV_INFINITE
                      : geo == "periodic" ? PV_PERIODIC
                      : geo == "disk"     ? PV_DISK
@@ -315,6 +335,8 @@ void check_rules(const pv_launch_rule *rules, size_t count) {
             raise(PV_ERR_INVALID_ARGUMENT, "source splits must be non-negative");
         if (r.tile != 0 && r.tile != 128 && r.tile != 256)
             raise(PV_ERR_INVALID_ARGUMENT, "source tile must be 0, 128 or 256");
+        if (r.symmetric != 0 && r.symmetric != 1)
+            raise(PV_ERR_INVALID_ARGUMENT, "symmetric must be 0 or 1");
     }
 }
 
@@ -344,12 +366,56 @@ std::vector<pv_launch_rule> default_launch_table() {
     return {};
 }
 
+void record_events(pv_handle *h, hipEvent_t &ev0, hipEvent_t &ev1) {
+    if (!h->profiling)
+        return;
+    while (h->events.size() < h->events_used + 2) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "create HIP event");
+        h->events.push_back(e);
+    }
+    ev0 = h->events[h->events_used++];
+    ev1 = h->events[h->events_used++];
+}
+
+// Whole-population evaluation over unordered tile pairs: the pair kernel writes every
+// (slot, vortex) partial once; the reduce kernel sums a vortex's slots in order and applies
+// the stage epilogue.  Both launches are bracketed by the profiling events.
+void evaluate_symmetric(pv_handle *h, const double *x, const double *y, const double *g,
+                        size_t n, const Shape &sh, const pv::StageOp &op) {
+    const size_t tt = (size_t)pv::kBlock * sh.tpt, nt = (n + tt - 1) / tt;
+    ensure_partials(h, nt * 2 * n);
+    pv::SymLaunch a{};
+    a.x = x, a.y = y, a.g = g;
+    a.n = (int)n;
+    a.nt = (int)nt;
+    a.part = h->part;
+    a.st = h->st;
+    a.epoch = ++h->epoch;
+    a.op = op;
+    h->records.push_back({x, y, (int)n, 0, (int)n});
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    record_events(h, ev0, ev1);
+    if (ev0)
+        hip_check(hipEventRecord(ev0, h->stream), "record HIP event");
+    pv::launch_sym(h->cfg.geometry, sh.tpt, sh.tile, a, h->gp, h->stream);
+    hip_check(hipGetLastError(), "launch HIP velocity kernel");
+    pv::launch_sym_reduce(h->cfg.geometry, a, h->gp, h->stream);
+    hip_check(hipGetLastError(), "launch HIP velocity reduction");
+    if (ev1)
+        hip_check(hipEventRecord(ev1, h->stream), "record HIP event");
+}
+
 // One velocity evaluation of positions (x, y) with circulations g, targets [tb, te).
 void evaluate(pv_handle *h, const double *x, const double *y, const double *g, size_t n,
               size_t tb, size_t te, const pv::StageOp &op) {
     const int prec = h->cfg.precision;
     const size_t targets = te - tb;
     const Shape sh = resolve_shape(h, targets, n);
+    if (sh.sym && tb == 0 && te == n) {
+        evaluate_symmetric(h, x, y, g, n, sh, op);
+        return;
+    }
     const int tpt = sh.tpt;
     const size_t tiles = sh.tiles, nsplit = sh.nsplit, split_len = sh.split_len;
     if (nsplit > 1)
@@ -369,16 +435,9 @@ void evaluate(pv_handle *h, const double *x, const double *y, const double *g, s
     a.op = op;
     h->records.push_back({x, y, (int)n, (int)tb, (int)te});
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (h->profiling) {
-        while (h->events.size() < h->events_used + 2) {
-            hipEvent_t e;
-            hip_check(hipEventCreate(&e), "create HIP event");
-            h->events.push_back(e);
-        }
-        ev0 = h->events[h->events_used++];
-        ev1 = h->events[h->events_used++];
+    record_events(h, ev0, ev1);
+    if (ev0)
         hip_check(hipEventRecord(ev0, h->stream), "record HIP event");
-    }
     pv::launch_pair(h->cfg.geometry, prec, tpt, sh.tile, a, h->gp, h->stream);
     hip_check(hipGetLastError(), "launch HIP velocity kernel");
     if (ev1)
@@ -614,19 +673,23 @@ pv_status pv_set_launch(pv_handle *h, int targets_per_thread, int source_splits)
         h->tpt_override = targets_per_thread;
         h->split_override = source_splits;
         h->tile_override = 0;
+        h->sym_override = -1;
     });
 }
 
 pv_status pv_set_launch_shape(pv_handle *h, int targets_per_thread, int source_splits,
-                              int tile) {
+                              int tile, int symmetric) {
     return guard([&] {
         checked(h);
+        if (symmetric < -1 || symmetric > 1)
+            raise(PV_ERR_INVALID_ARGUMENT, "symmetric must be -1, 0 or 1");
         const pv_launch_rule r{h->cfg.geometry, h->cfg.precision, 0, targets_per_thread,
-                               source_splits, tile};
+                               source_splits, tile, symmetric < 0 ? 0 : symmetric};
         check_rules(&r, 1);
         h->tpt_override = targets_per_thread;
         h->split_override = source_splits;
         h->tile_override = tile;
+        h->sym_override = symmetric;
     });
 }
 
@@ -641,15 +704,16 @@ pv_status pv_set_launch_table(pv_handle *h, const pv_launch_rule *rules, size_t 
 }
 
 pv_status pv_resolve_launch(pv_handle *h, size_t targets, size_t n, int *targets_per_thread,
-                            int *source_splits, int *tile) {
+                            int *source_splits, int *tile, int *symmetric) {
     return guard([&] {
         checked(h);
-        if (!targets_per_thread || !source_splits || !tile)
+        if (!targets_per_thread || !source_splits || !tile || !symmetric)
             raise(PV_ERR_INVALID_ARGUMENT, "null argument");
         const Shape s = resolve_shape(h, targets, n);
         *targets_per_thread = s.tpt;
-        *source_splits = (int)s.nsplit;
+        *source_splits = s.sym ? 0 : (int)s.nsplit;
         *tile = s.tile;
+        *symmetric = s.sym ? 1 : 0;
     });
 }
 

@@ -55,23 +55,29 @@ __device__ __forceinline__ double rcp64(double a) {
 
 __device__ __forceinline__ float rcp32(float a) { return __builtin_amdgcn_rcpf(a); }
 
-// Reciprocals of K positive values with one rcp64 (K a power of two): a product tree, one
-// refined reciprocal of the root, then back-multiplication down the tree; about 2 log2(K) ulps.
+// Reciprocals of K positive values with one rcp64: a product tree (an odd element at a level
+// is passed up unpaired), one refined reciprocal of the root, then back-multiplication down the
+// tree; about 2 log2(K) ulps.
 // Returns false (inverses then unusable) when the root leaves [2^-960, 2^960], where a partial
 // product could have lost bits; callers then fall back to one rcp64 per value.
 template <int K> struct TreeInverse {
     __device__ static __forceinline__ bool run(const double *d, double *inv) {
-        static_assert((K & (K - 1)) == 0 && K >= 2, "K must be a power of two");
-        double lo[K / 2], lo_inv[K / 2];
+        static_assert(K >= 2, "K must be at least 2");
+        constexpr int H = K / 2, R = K - 2 * H;   // pairs, and an odd element passed up as is
+        double lo[H + R], lo_inv[H + R];
 #pragma unroll
-        for (int k = 0; k < K / 2; ++k)
+        for (int k = 0; k < H; ++k)
             lo[k] = d[2 * k] * d[2 * k + 1];
-        const bool ok = TreeInverse<K / 2>::run(lo, lo_inv);
+        if constexpr (R)
+            lo[H] = d[K - 1];
+        const bool ok = TreeInverse<H + R>::run(lo, lo_inv);
 #pragma unroll
-        for (int k = 0; k < K / 2; ++k) {
+        for (int k = 0; k < H; ++k) {
             inv[2 * k] = lo_inv[k] * d[2 * k + 1];
             inv[2 * k + 1] = lo_inv[k] * d[2 * k];
         }
+        if constexpr (R)
+            inv[K - 1] = lo_inv[H];
         return ok;
     }
 };
@@ -279,6 +285,25 @@ struct InfF64 {
         a.u = __builtin_fma(-c, m.dy, a.u);
         a.v = __builtin_fma(c, m.dx, a.v);
     }
+    // Symmetric evaluation (each unordered pair once): the term on j from i reuses the
+    // difference and the inverse with i's circulation and the opposite sign of (dx, dy).
+    static constexpr bool kSym = true;
+    static constexpr int kSymDenoms = 1;
+    using SymTgt = Tgt;
+    using SymTmp = Tmp;
+    __device__ static double sym_g(const GeoParams &, double g) { return kInvTwoPi * g; }
+    __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
+    __device__ static void sym_prep(const GeoParams &p, const SymTgt &t, const Src &s, SymTmp &m,
+                                    double *d) {
+        prep(p, t, s, m, d);
+    }
+    __device__ static void sym_apply(const Src &s, double gi, const SymTmp &m, const double *inv,
+                                     Acc &a, Acc &r) {
+        apply(s, m, inv, a);
+        const double c = gi * inv[0];
+        r.u = __builtin_fma(c, m.dy, r.u);
+        r.v = __builtin_fma(-c, m.dx, r.v);
+    }
 };
 
 // Mixed: FP64 difference, FP32 1/r^2 and products, FP32 tile sums folded into FP64.
@@ -298,6 +323,7 @@ struct InfMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
+    static constexpr bool kSym = false;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
     }
@@ -438,6 +464,49 @@ struct DiskF64 {
         a.u = __builtin_fma(-k, m.ey, a.u);
         a.v = __builtin_fma(k, m.ex, a.v);
     }
+    // Symmetric evaluation: the direct term is shared between (i, j) and (j, i); each side
+    // still gets its own image term (j's image at i, i's image at j).  The target carries its
+    // own image point.  Blocks holding a near-centre vortex use the reference's form both ways.
+    static constexpr bool kSym = true;
+    static constexpr int kSymDenoms = 3;
+    struct SymTgt {
+        double x, y, tx, ty, ix, iy;
+    };
+    struct SymTmp {
+        double dx, dy, ex, ey, fx, fy;     // x_i - x_j, x_i - x_j*, x_j - x_i*
+    };
+    __device__ static double sym_g(const GeoParams &, double g) { return kInvTwoPi * g; }
+    __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) {
+        const Src s = src(p, x, y, 0.0);
+        return {x, y, x * p.invR, y * p.invR, s.ix, s.iy};
+    }
+    __device__ static void sym_prep(const GeoParams &, const SymTgt &t, const Src &s, SymTmp &m,
+                                    double *d) {
+        m.dx = t.x - s.x;
+        m.dy = t.y - s.y;
+        d[0] = __builtin_fma(m.dx, m.dx, m.dy * m.dy);
+        m.ex = t.x - s.ix;
+        m.ey = t.y - s.iy;
+        d[1] = __builtin_fma(m.ex, m.ex, m.ey * m.ey);
+        m.fx = s.x - t.ix;
+        m.fy = s.y - t.iy;
+        d[2] = __builtin_fma(m.fx, m.fx, m.fy * m.fy);
+    }
+    __device__ static void sym_apply(const Src &s, double gi, const SymTmp &m, const double *inv,
+                                     Acc &a, Acc &r) {
+        const double c = s.g * inv[0];
+        a.u = __builtin_fma(-c, m.dy, a.u);
+        a.v = __builtin_fma(c, m.dx, a.v);
+        const double k = s.ig * inv[1];
+        a.u = __builtin_fma(-k, m.ey, a.u);
+        a.v = __builtin_fma(k, m.ex, a.v);
+        const double ci = gi * inv[0];
+        r.u = __builtin_fma(ci, m.dy, r.u);
+        r.v = __builtin_fma(-ci, m.dx, r.v);
+        const double ki = -gi * inv[2];            // i's image has strength -G_i
+        r.u = __builtin_fma(-ki, m.fy, r.u);
+        r.v = __builtin_fma(ki, m.fx, r.v);
+    }
 };
 
 // Mixed disk.  The direct term as in the infinite plane (FP64 difference d = x_i - x_j, FP32
@@ -463,6 +532,7 @@ struct DiskMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
+    static constexpr bool kSym = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         const double r2 = __builtin_fma(x, x, y * y);
         float ix = 1e30f, iy = 0.f;
@@ -602,6 +672,25 @@ struct PerF64 {
         a.u = __builtin_fma(-s.gs, fu, a.u);
         a.v = __builtin_fma(s.gs, fv, a.v);
     }
+    // Symmetric evaluation: with X, Y -> -X, -Y every factor is odd or even, so the term on j
+    // from i is the same (fu, fv) with i's circulation and the opposite sign.
+    static constexpr bool kSym = true;
+    static constexpr int kSymDenoms = 0;
+    using SymTgt = Tgt;
+    struct SymTmp {};
+    __device__ static double sym_g(const GeoParams &p, double g) { return p.scale * g; }
+    __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
+    __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, double gi, const Src &s,
+                                    Acc &a, Acc &r) {
+        double tu, tv, d0u, d0v, sx, sy;
+        terms(p, t, s, tu, tv, d0u, d0v, sx, sy);
+        const double fu = __builtin_fma(sy, rcp64(d0u), tu);
+        const double fv = __builtin_fma(sx, rcp64(d0v), tv);
+        a.u = __builtin_fma(-s.gs, fu, a.u);
+        a.v = __builtin_fma(s.gs, fv, a.v);
+        r.u = __builtin_fma(gi, fu, r.u);
+        r.v = __builtin_fma(-gi, fv, r.v);
+    }
     __device__ static void fold(Acc &) {}
     __device__ static void result(const Acc &a, double &u, double &v) {
         u = a.u;
@@ -625,6 +714,7 @@ struct PerMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
+    static constexpr bool kSym = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
     }

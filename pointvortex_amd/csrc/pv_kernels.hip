@@ -196,6 +196,267 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
     }
 }
 
+// ------------------------------------------------------------------ symmetric evaluation
+
+// Sum of (r.u, r.v) over the 64 lanes of a wave; every lane gets the same bits (an xor
+// butterfly adds the same two values on both partners at every level).
+template <class A> __device__ __forceinline__ void wave_sum(A &r) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        r.u += __shfl_xor(r.u, off, 64);
+        r.v += __shfl_xor(r.v, off, 64);
+    }
+}
+
+// Block -> tile pair (I <= J) in row-major triangular order: b = J (J + 1) / 2 + I.
+__device__ __forceinline__ void tile_pair(int b, int &I, int &J) {
+    int j = (int)((__builtin_sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((j + 1) * (j + 2) / 2 <= b)
+        ++j;
+    while (j * (j + 1) / 2 > b)
+        --j;
+    J = j;
+    I = b - j * (j + 1) / 2;
+}
+
+template <class P, int TPT, int TILE>
+__global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) {
+    if (a.st->fail_epoch < a.epoch)
+        return;
+    using Acc = typename P::Acc;
+    __shared__ typename P::Src sh[TILE];
+    __shared__ double red[kBlock / 64][TILE][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int TT = kBlock * TPT;
+    int I, J;
+    tile_pair(blockIdx.x, I, J);
+    const int t0 = I * TT;
+    const int s0 = J * TT, s1 = min(a.n, s0 + TT);
+    const size_t n = (size_t)a.n;
+
+    double xi[TPT], yi[TPT], gi[TPT];
+    int ti[TPT];
+    int near = 0;
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = t0 + t * kBlock + tid;
+        const bool valid = i < a.n;
+        const int ii = valid ? i : t0;
+        xi[t] = a.x[ii];
+        yi[t] = a.y[ii];
+        gi[t] = valid ? P::sym_g(gp, a.g[ii]) : 0.0;     // a padding target adds no reaction
+        ti[t] = valid ? i : -1;
+        if constexpr (P::kNearTiles)
+            near |= valid && P::is_near(gp, xi[t], yi[t]);
+    }
+    if constexpr (P::kNearTiles) {
+        for (int k = tid; k < s1 - s0; k += kBlock)
+            near |= P::is_near(gp, a.x[s0 + k], a.y[s0 + k]);
+        near = __syncthreads_or(near);
+    }
+
+    Acc acc[TPT];
+#pragma unroll
+    for (int t = 0; t < TPT; ++t)
+        P::zero(acc[t]);
+
+    if (I == J) {
+        // Ordered pairs inside the tile, the target side only (self pair skipped).
+        typename P::Tgt tg[TPT];
+#pragma unroll
+        for (int t = 0; t < TPT; ++t)
+            tg[t] = P::tgt(gp, xi[t], yi[t]);
+        for (int base = s0; base < s1; base += TILE) {
+            const int cnt = min(TILE, s1 - base);
+            __syncthreads();
+            for (int k = tid; k < cnt; k += kBlock)
+                sh[k] = P::src(gp, a.x[base + k], a.y[base + k], a.g[base + k]);
+            __syncthreads();
+            for (int k = 0; k < cnt; ++k) {
+                const typename P::Src s_a = sh[k];
+#pragma unroll
+                for (int t = 0; t < TPT; ++t) {
+                    if constexpr (P::kNearTiles) {
+                        if (near)
+                            P::pair_ref_checked(gp, tg[t], s_a, acc[t], ti[t] == base + k);
+                        else
+                            P::pair_checked(gp, tg[t], s_a, acc[t], ti[t] == base + k);
+                    } else {
+                        P::pair_checked(gp, tg[t], s_a, acc[t], ti[t] == base + k);
+                    }
+                }
+            }
+        }
+    } else {
+        typename P::SymTgt tg[TPT];
+#pragma unroll
+        for (int t = 0; t < TPT; ++t)
+            tg[t] = P::sym_tgt(gp, xi[t], yi[t]);
+        for (int base = s0; base < s1; base += TILE) {
+            const int cnt = min(TILE, s1 - base);
+            __syncthreads();
+            for (int k = tid; k < cnt; k += kBlock)
+                sh[k] = P::src(gp, a.x[base + k], a.y[base + k], a.g[base + k]);
+            __syncthreads();
+            bool done = false;
+            if constexpr (P::kNearTiles) {
+                if (near) {
+                    // The reference's form both ways: j on i, and i (as a source) on j.
+                    typename P::Tgt tr[TPT];
+                    typename P::Src si[TPT];
+#pragma unroll
+                    for (int t = 0; t < TPT; ++t) {
+                        tr[t] = P::tgt(gp, xi[t], yi[t]);
+                        si[t] = P::src(gp, xi[t], yi[t], ti[t] >= 0 ? a.g[ti[t]] : 0.0);
+                    }
+                    for (int k = 0; k < cnt; ++k) {
+                        const typename P::Src s_a = sh[k];
+                        const typename P::Tgt tj = P::tgt(gp, s_a.x, s_a.y);
+                        Acc r;
+                        P::zero(r);
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t) {
+                            P::pair_ref_checked(gp, tr[t], s_a, acc[t], false);
+                            P::pair_ref_checked(gp, tj, si[t], r, false);
+                        }
+                        wave_sum(r);
+                        if (lane == 0) {
+                            red[wave][k][0] = r.u;
+                            red[wave][k][1] = r.v;
+                        }
+                    }
+                    done = true;
+                }
+            }
+            if (!done) {
+                if constexpr (P::kSymDenoms > 0) {
+                    constexpr int ND = P::kSymDenoms;
+                    int k = 0;
+                    for (; k + 2 <= cnt; k += 2) {
+                        const typename P::Src s2[2] = {sh[k], sh[k + 1]};
+                        typename P::SymTmp m[2][TPT];
+                        double d[2 * TPT * ND], inv[2 * TPT * ND];
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+#pragma unroll
+                            for (int t = 0; t < TPT; ++t)
+                                P::sym_prep(gp, tg[t], s2[q], m[q][t], &d[(q * TPT + t) * ND]);
+                        const bool ok = TreeInverse<2 * TPT * ND>::run(d, inv);
+                        if (!__all(ok)) {
+#pragma unroll
+                            for (int e = 0; e < 2 * TPT * ND; ++e)
+                                inv[e] = rcp64(d[e]);
+                        }
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            Acc r;
+                            P::zero(r);
+#pragma unroll
+                            for (int t = 0; t < TPT; ++t)
+                                P::sym_apply(s2[q], gi[t], m[q][t], &inv[(q * TPT + t) * ND],
+                                             acc[t], r);
+                            wave_sum(r);
+                            if (lane == 0) {
+                                red[wave][k + q][0] = r.u;
+                                red[wave][k + q][1] = r.v;
+                            }
+                        }
+                    }
+                    if (k < cnt) {
+                        const typename P::Src s_a = sh[k];
+                        typename P::SymTmp m[TPT];
+                        double d[TPT * ND], inv[TPT * ND];
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t)
+                            P::sym_prep(gp, tg[t], s_a, m[t], &d[t * ND]);
+#pragma unroll
+                        for (int e = 0; e < TPT * ND; ++e)
+                            inv[e] = rcp64(d[e]);
+                        Acc r;
+                        P::zero(r);
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t)
+                            P::sym_apply(s_a, gi[t], m[t], &inv[t * ND], acc[t], r);
+                        wave_sum(r);
+                        if (lane == 0) {
+                            red[wave][k][0] = r.u;
+                            red[wave][k][1] = r.v;
+                        }
+                    }
+                } else {
+                    for (int k = 0; k < cnt; ++k) {
+                        const typename P::Src s_a = sh[k];
+                        Acc r;
+                        P::zero(r);
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t)
+                            P::sym_pair(gp, tg[t], gi[t], s_a, acc[t], r);
+                        wave_sum(r);
+                        if (lane == 0) {
+                            red[wave][k][0] = r.u;
+                            red[wave][k][1] = r.v;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // The reaction on this chunk's sources: the four waves in order, into slot I.
+            for (int k = tid; k < cnt; k += kBlock) {
+                double ru = red[0][k][0], rv = red[0][k][1];
+#pragma unroll
+                for (int w = 1; w < kBlock / 64; ++w) {
+                    ru += red[w][k][0];
+                    rv += red[w][k][1];
+                }
+                a.part[(size_t)(2 * I) * n + base + k] = ru;
+                a.part[(size_t)(2 * I + 1) * n + base + k] = rv;
+            }
+        }
+    }
+    // The target side into slot J (slot I on the diagonal, where J == I).
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        if (ti[t] >= 0) {
+            double u, v;
+            P::result(acc[t], u, v);
+            a.part[(size_t)(2 * J) * n + ti[t]] = u;
+            a.part[(size_t)(2 * J + 1) * n + ti[t]] = v;
+        }
+    }
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void sym_reduce_kernel(SymLaunch a, GeoParams gp) {
+    if (a.st->fail_epoch < a.epoch)
+        return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double err = 0.0;
+    if (i < a.n) {
+        const double x = a.x[i], y = a.y[i];
+        // src/vortex.h:11-13 and src/compute.cpp:41-48 for every evaluated target.
+        if (!finite64(x) || !finite64(y))
+            record_failure(a.st, a.epoch, kFailStateNonFinite);
+        else if (gp.geometry == 2 && x * x + y * y >= gp.R2)
+            record_failure(a.st, a.epoch, kFailOutsideDisk);
+        const size_t n = (size_t)a.n;
+        Comp su, sv;
+        su.zero();
+        sv.zero();
+        for (int s = 0; s < a.nt; ++s) {
+            su.add(__builtin_nontemporal_load(&a.part[(size_t)(2 * s) * n + i]));
+            sv.add(__builtin_nontemporal_load(&a.part[(size_t)(2 * s + 1) * n + i]));
+        }
+        err = stage_epilogue(a.op, a.op.dt, a.op.atol, a.op.rtol,
+                             FailSink{a.st, a.epoch, nullptr}, a.x, a.y, i, su.value(),
+                             sv.value());
+    }
+    if (a.op.kind == kOpDopriError) {
+        const double m = block_max(err);
+        if (threadIdx.x == 0)
+            atomicMax(&a.st->err_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
 // ------------------------------------------------------------------ small kernels
 
 __global__ void reset_status_kernel(DevStatus *st) {
@@ -520,6 +781,51 @@ void launch_pair(int geometry, int precision, int tpt, int tile, const PairLaunc
         else
             launch_pair_tile<DiskMixed>(tpt, tile, a, gp, s);
     }
+}
+
+bool sym_supported(int geometry, int precision) {
+    (void)geometry;
+    return precision == 0;
+}
+
+template <class P, int TILE>
+static void launch_sym_tpt(int tpt, const SymLaunch &a, const GeoParams &gp, hipStream_t s) {
+    const dim3 grid((unsigned)((size_t)a.nt * (a.nt + 1) / 2));
+    if (tpt >= 4)
+        hipLaunchKernelGGL((sym_kernel<P, 4, TILE>), grid, dim3(kBlock), 0, s, a, gp);
+    else if (tpt == 2)
+        hipLaunchKernelGGL((sym_kernel<P, 2, TILE>), grid, dim3(kBlock), 0, s, a, gp);
+    else
+        hipLaunchKernelGGL((sym_kernel<P, 1, TILE>), grid, dim3(kBlock), 0, s, a, gp);
+}
+
+template <class P>
+static void launch_sym_p(int tpt, int tile, const SymLaunch &a, const GeoParams &gp,
+                         hipStream_t s) {
+    if (tile == 128)
+        launch_sym_tpt<P, 128>(tpt, a, gp, s);
+    else
+        launch_sym_tpt<P, 256>(tpt, a, gp, s);
+}
+
+void launch_sym(int geometry, int tpt, int tile, const SymLaunch &a, const GeoParams &gp,
+                hipStream_t s) {
+    if (geometry == 0)
+        launch_sym_p<InfF64>(tpt, tile, a, gp, s);
+    else if (geometry == 1)
+        launch_sym_p<PerF64>(tpt, tile, a, gp, s);
+    else
+        launch_sym_p<DiskF64>(tpt, tile, a, gp, s);
+}
+
+void launch_sym_reduce(int geometry, const SymLaunch &a, const GeoParams &gp, hipStream_t s) {
+    const dim3 grid((unsigned)((a.n + kBlock - 1) / kBlock));
+    if (geometry == 0)
+        hipLaunchKernelGGL((sym_reduce_kernel<InfF64>), grid, dim3(kBlock), 0, s, a, gp);
+    else if (geometry == 1)
+        hipLaunchKernelGGL((sym_reduce_kernel<PerF64>), grid, dim3(kBlock), 0, s, a, gp);
+    else
+        hipLaunchKernelGGL((sym_reduce_kernel<DiskF64>), grid, dim3(kBlock), 0, s, a, gp);
 }
 
 void launch_make_stage(const StageOp &op, int n, const DevStatus *st, unsigned epoch,

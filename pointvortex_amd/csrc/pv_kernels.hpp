@@ -52,6 +52,22 @@ struct PairLaunch {
     StageOp op;
 };
 
+// Symmetric evaluation of a whole population (FP64 policies with kSym): one block per
+// unordered pair of tiles (I <= J) of kBlock*TPT vortices.  Off-diagonal blocks compute each
+// pair once and write the target side (into tile I's vortices, slot J) and the reaction (into
+// tile J's vortices, slot I); diagonal blocks compute the ordered pairs inside their tile (slot
+// I).  Every (slot, vortex) entry of `part` is written exactly once; the reduce kernel then sums
+// a vortex's nt slots in slot order and applies the stage epilogue.
+struct SymLaunch {
+    const double *x, *y, *g;
+    int n;
+    int nt;                    // tiles of kBlock*TPT vortices
+    double *part;              // [nt][2][n]
+    DevStatus *st;
+    unsigned epoch;
+    StageOp op;
+};
+
 constexpr int kBlock = 256;
 constexpr int kTileF64 = 256;
 constexpr int kTileMixed = 128;
@@ -61,6 +77,11 @@ constexpr int kTileMixed = 128;
 void launch_pair(int geometry, int precision, int tpt, int tile, const PairLaunch &a,
                  const GeoParams &gp, hipStream_t s);
 int pair_tile(int precision);
+// Symmetric path: supported for the FP64 policies; two launches (pairs, then reduce+epilogue).
+bool sym_supported(int geometry, int precision);
+void launch_sym(int geometry, int tpt, int tile, const SymLaunch &a, const GeoParams &gp,
+                hipStream_t s);
+void launch_sym_reduce(int geometry, const SymLaunch &a, const GeoParams &gp, hipStream_t s);
 void launch_make_stage(const StageOp &op, int n, const DevStatus *st, unsigned epoch,
                        hipStream_t s);
 void launch_reset_status(DevStatus *st, hipStream_t s);

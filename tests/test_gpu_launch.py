@@ -51,11 +51,16 @@ def test_table_precedence():
                         ("disk", "fp64", 1000, 4, 3, 128),          # N too large for it
                         ("disk", "fp64", 10**6, 1, 2, 128),         # this one
                         ("disk", "fp64", 10**9, 4, 7, 256)])
-    assert k.resolved_launch(pop.n, pop.n) == (1, 2, 128)
+    assert k.resolved_launch(pop.n, pop.n) == (1, 2, 128, 0)
     k.set_launch(4, 0, 256)                                          # explicit override wins
     assert k.resolved_launch(pop.n, pop.n)[0] == 4 and k.resolved_launch(pop.n, pop.n)[2] == 256
     k.set_launch(0, 0, 0)
-    assert k.resolved_launch(pop.n, pop.n) == (1, 2, 128)
+    assert k.resolved_launch(pop.n, pop.n) == (1, 2, 128, 0)
+    k.set_launch_table([("disk", "fp64", 10**6, 2, 0, 256, 1)])
+    assert k.resolved_launch(pop.n, pop.n) == (2, 0, 256, 1)        # symmetric rule
+    assert k.resolved_launch(100, pop.n)[3] == 0                     # subsets: split path
+    k.set_launch(0, 0, 0, 0)
+    assert k.resolved_launch(pop.n, pop.n)[3] == 0                   # forced off
     k.set_launch_table([])
     assert k.resolved_launch(pop.n, pop.n) == default
     with pytest.raises(InvalidArgument, match="tile"):
@@ -71,7 +76,7 @@ def test_table_from_environment(tmp_path):
     try:
         os.environ["PV_HIP_LAUNCH_TABLE"] = str(table)
         k = HipKernel(SimParams(boundaryCondition="disk"))
-        assert k.resolved_launch(5000, 5000) == (4, 3, 128)
+        assert k.resolved_launch(5000, 5000) == (4, 3, 128, 0)
         os.environ["PV_HIP_LAUNCH_TABLE"] = str(bad)
         with pytest.raises(InvalidArgument, match="malformed"):
             HipKernel(SimParams(boundaryCondition="disk"))
@@ -89,13 +94,68 @@ def test_table_from_environment(tmp_path):
 def test_autotuner_choice_keeps_parity(oracle, geometry, precision):
     res = autotune.tune_one(geometry, precision, 2048, log=lambda *a: None)
     assert res["best_ms"] <= res["default_ms"] * (1.0 + autotune.TIE)
-    tpt, splits, tile = res["best"]
+    tpt, splits, tile, sym = res["best"]
     pop = autotune.population(geometry, 2048)
     k = HipKernel(SimParams(boundaryCondition=geometry, precision=precision,
                             boxLengthX=2 * np.pi, boxLengthY=2 * np.pi))
-    k.set_launch(tpt, splits, tile)
+    k.set_launch(tpt, splits, tile, sym)
     vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
     ur, vr = oracle.evaluate(spec(geometry, box_length=2 * np.pi), pop.x, pop.y, pop.circulation)
     gate = (fp64_gate(geometry, pop.x, pop.y, pop.circulation, ur, vr) if precision == "fp64"
             else 1e-6)
     assert normwise(vel.x, vel.y, ur, vr) <= gate
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+@pytest.mark.parametrize("n", [2, 700, 1500, 5000])
+@pytest.mark.parametrize("tpt,tile", [(1, 128), (2, 256), (4, 128)])
+def test_symmetric_path_keeps_parity(oracle, geometry, n, tpt, tile):
+    """Each unordered pair once over pairs of tiles (N not a multiple of the tile included)."""
+    pop, k = setup(geometry, "fp64", n, seed=5 + n)
+    k.set_launch(tpt, 0, tile, 1)
+    assert k.resolved_launch(pop.n, pop.n) == (tpt, 0, tile, 1)
+    vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+    ur, vr = oracle.evaluate(spec(geometry, box_length=2.0), pop.x, pop.y, pop.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= fp64_gate(geometry, pop.x, pop.y, pop.circulation,
+                                                       ur, vr)
+
+
+def test_symmetric_path_near_centre_disk_source(oracle):
+    pop, k = setup("disk", "fp64", 1800, seed=71)
+    x, y = pop.x.copy(), pop.y.copy()
+    x[1300], y[1300] = 2e-160, 1e-161
+    k.set_launch(2, 0, 128, 1)
+    vel = k.evaluate(VortexSystem(x, y, pop.circulation))
+    ur, vr = oracle.evaluate(spec("disk"), x, y, pop.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= fp64_gate("disk", x, y, pop.circulation, ur, vr)
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_symmetric_path_in_rk4_and_fallbacks(oracle, geometry):
+    """RK4 steps through the symmetric path agree with the split path; mixed precision and
+    target subsets fall back to the split path; repeats are bit-identical."""
+    pop, ks = setup(geometry, "fp64", 3000, seed=12)
+    _, kp = setup(geometry, "fp64", 3000, seed=12)
+    ks.set_launch(2, 0, 256, 1)
+    kp.set_launch(2, 0, 256, 0)
+    s = VortexSystem(pop.x, pop.y, pop.circulation)
+    a, b = ks.evaluate(s), ks.evaluate(s)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    for kern in (ks, kp):
+        kern.uploadDeviceState(s)
+        for _ in range(3):
+            kern.deviceRk4Step(1e-4)
+    es, ep = s.copy(), s.copy()
+    ks.downloadDeviceState(es)
+    kp.downloadDeviceState(ep)
+    scale = np.max(np.abs(np.concatenate([ep.x, ep.y])))
+    assert np.max(np.abs(es.x - ep.x)) <= 1e-12 * scale
+    assert np.max(np.abs(es.y - ep.y)) <= 1e-12 * scale
+    assert ks.resolved_launch(100, 3000)[3] == 0
+    sub = ks.evaluateRange(pop.x, pop.y, pop.circulation, 100, 400)
+    ur, vr = oracle.evaluate_range(spec(geometry, box_length=2.0), pop.x, pop.y,
+                                   pop.circulation, 100, 400)
+    assert normwise(sub.x[100:400], sub.y[100:400], ur[100:400], vr[100:400]) <= 1e-12
+    _, km = setup(geometry, "mixed", 3000, seed=12)
+    km.set_launch(2, 0, 256, 1)
+    assert km.resolved_launch(3000, 3000)[3] == 0
