@@ -164,8 +164,8 @@ pv_status pv_synchronize(pv_handle *h);
 /* Launch shapes of the pair kernels (no reference counterpart; the CUDA backend used one
  * fixed shape, src/backend_cuda.cu:402-416).  A shape is targets per thread (1, 2 or 4; 0 =
  * automatic), source splits (0 = automatic), the LDS source tile (128 or 256; 0 = the
- * precision's default) and whether whole-population FP64 evaluations use the symmetric path
- * (each unordered pair computed once; 0 or 1).  Precedence: pv_set_launch /
+ * precision's default) and whether whole-population evaluations use the symmetric path (each
+ * unordered pair computed once; 0 or 1; both precisions).  Precedence: pv_set_launch /
  * pv_set_launch_shape (symmetric -1 = not forced), then the first matching rule of the handle's
  * launch table, then a built-in heuristic (split path).  A new handle's table is read from the
  * file named by $PV_HIP_LAUNCH_TABLE, else from launch_table.txt beside libpvhip.so when present

@@ -1,8 +1,8 @@
 """Autotuner of the pair-kernel launch shape per (geometry, precision, N-bucket).
 
 A shape is (targets per thread, source splits, LDS source tile, symmetric); its blocks per CU
-follow from the split count.  The symmetric path (FP64 only) computes each unordered pair once
-over pairs of tiles; its candidates vary targets per thread and the tile.  For every geometry, precision and N-bucket the tuner uploads a seeded
+follow from the split count.  The symmetric path computes each unordered pair once over pairs
+of tiles; its candidates vary targets per thread and the tile.  For every geometry, precision and N-bucket the tuner uploads a seeded
 population at the bucket's upper N, evaluates it once with the built-in shape (the
 reference result), then for every distinct candidate shape:
 
@@ -113,8 +113,7 @@ def tune_one(geometry, precision, n, log=_log):
     rejected = 0
     candidates = [(tpt, tile, splits, 0) for tpt, tile, splits in
                   itertools.product(TPT, TILES, SPLITS)]
-    if precision == "fp64":
-        candidates += [(tpt, tile, 0, 1) for tpt, tile in itertools.product(TPT, TILES)]
+    candidates += [(tpt, tile, 0, 1) for tpt, tile in itertools.product(TPT, TILES)]
     for tpt, tile, splits, sym in candidates:
         k.set_launch(tpt, splits, tile, sym)
         shape = k.resolved_launch(n, n)

@@ -315,7 +315,7 @@ class HipKernel:
 
     def set_launch(self, targets_per_thread=0, source_splits=0, tile=0, symmetric=-1):
         """Explicit launch shape of the pair kernels (0 = automatic / default; symmetric -1 =
-        not forced, 0 = split path, 1 = symmetric path for whole-population FP64 evaluations)."""
+        not forced, 0 = split path, 1 = symmetric path for whole-population evaluations)."""
         _check(_lib.pv_set_launch_shape(self._h, int(targets_per_thread), int(source_splits),
                                         int(tile), int(symmetric)))
 

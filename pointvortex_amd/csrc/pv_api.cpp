@@ -219,7 +219,7 @@ void begin_sequence(pv_handle *h) {
 struct Shape {
     int tpt, tile;
     size_t tiles, nsplit, split_len;
-    bool sym;      // symmetric path (whole-population FP64 evaluations only)
+    bool sym;      // symmetric path (whole-population evaluations only)
 };
 
 // Partial sums of the symmetric path are nt x 2N doubles; beyond this the split path runs.
@@ -398,9 +398,9 @@ void evaluate_symmetric(pv_handle *h, const double *x, const double *y, const do
     record_events(h, ev0, ev1);
     if (ev0)
         hip_check(hipEventRecord(ev0, h->stream), "record HIP event");
-    pv::launch_sym(h->cfg.geometry, sh.tpt, sh.tile, a, h->gp, h->stream);
+    pv::launch_sym(h->cfg.geometry, h->cfg.precision, sh.tpt, sh.tile, a, h->gp, h->stream);
     hip_check(hipGetLastError(), "launch HIP velocity kernel");
-    pv::launch_sym_reduce(h->cfg.geometry, a, h->gp, h->stream);
+    pv::launch_sym_reduce(a, h->gp, h->stream);
     hip_check(hipGetLastError(), "launch HIP velocity reduction");
     if (ev1)
         hip_check(hipEventRecord(ev1, h->stream), "record HIP event");

@@ -87,7 +87,7 @@ __device__ void batch_item(const BatchLaunch &a, const Instance *ins, const Work
                     P::prep(gp, tg, s2[0], m[0], &d[0]);
                     P::prep(gp, tg, s2[1], m[1], &d[ND]);
                     const bool ok = TreeInverse<K>::run(d, inv);
-                    if (!__all(ok)) {
+                    if (tree_fallback(ok)) {
 #pragma unroll
                         for (int e = 0; e < K; ++e)
                             inv[e] = rcp64(d[e]);

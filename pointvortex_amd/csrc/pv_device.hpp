@@ -88,6 +88,12 @@ template <> struct TreeInverse<1> {
     }
 };
 
+// True when the product tree left some lane of the wave out of range (a wave-uniform
+// condition: the ballot of the failing lanes is a scalar mask).
+__device__ __forceinline__ bool tree_fallback(bool ok) {
+    return __builtin_amdgcn_ballot_w64(!ok) != 0;
+}
+
 // Error-free addition (Knuth's TwoSum): s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
     s = a + b;
@@ -291,6 +297,8 @@ struct InfF64 {
     static constexpr int kSymDenoms = 1;
     using SymTgt = Tgt;
     using SymTmp = Tmp;
+    using SymScalar = double;
+    using SymR = Acc;
     __device__ static double sym_g(const GeoParams &, double g) { return kInvTwoPi * g; }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
     __device__ static void sym_prep(const GeoParams &p, const SymTgt &t, const Src &s, SymTmp &m,
@@ -323,7 +331,6 @@ struct InfMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
-    static constexpr bool kSym = false;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
     }
@@ -358,6 +365,30 @@ struct InfMixed {
         u = a.u.value();
         v = a.v.value();
     }
+    // Symmetric evaluation: one FP64 difference, one FP32 reciprocal per unordered pair; the
+    // reaction on the source is an FP32 sum (over at most 64 x 4 targets) like the target
+    // side's per-tile FP32 sums, and is added to the FP64 partials after that.
+    static constexpr bool kSym = true;
+    static constexpr int kSymDenoms = 0;
+    using SymTgt = Tgt;
+    struct SymTmp {};
+    using SymScalar = float;
+    struct SymR {
+        float u, v;
+    };
+    __device__ static float sym_g(const GeoParams &, double g) { return (float)(kInvTwoPi * g); }
+    __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
+    __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, float gi, const Src &s,
+                                    Acc &a, SymR &r) {
+        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+        const float inv = rcp32(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, (float)p.eps2)));
+        const float c = s.g * inv;
+        a.u32 = __builtin_fmaf(-c, dy, a.u32);
+        a.v32 = __builtin_fmaf(c, dx, a.v32);
+        const float ci = gi * inv;
+        r.u = __builtin_fmaf(ci, dy, r.u);
+        r.v = __builtin_fmaf(-ci, dx, r.v);
+    }
 };
 
 // ---------------------------------------------------------------- disk
@@ -374,8 +405,11 @@ struct InfMixed {
 // 2^-100 R of the centre has its image point near overflow; a tile holding one is computed
 // with the reference's a, b form instead (the tile flag is uniform, so no per-pair branch).
 struct DiskF64 {
+    // g = G/(2 pi); (ix, iy) the image point x R^2/|x|^2 (strength -g); w = 1/|x|^2.  A source
+    // within 2^-100 R of the centre has no finite image: (ix, iy, w) = 0, and every tile that
+    // holds one is evaluated in the reference's form instead.
     struct Src {
-        double x, y, g, ix, iy, ig;    // g = G/(2 pi); image point and strength -g
+        double x, y, g, ix, iy, w;
     };
     struct Tgt {
         double x, y, tx, ty;
@@ -395,7 +429,7 @@ struct DiskF64 {
         if (r2 < 0x1p-200 * p.R2)
             return {x, y, gg, 0.0, 0.0, 0.0};
         const double q = p.R2 / r2;
-        return {x, y, gg, x * q, y * q, -gg};
+        return {x, y, gg, x * q, y * q, 1.0 / r2};
     }
     __device__ static Tgt tgt(const GeoParams &p, double x, double y) {
         return {x, y, x * p.invR, y * p.invR};
@@ -408,7 +442,7 @@ struct DiskF64 {
     }
     __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, Acc &a) {
         point(t.x - s.x, t.y - s.y, s.g, a);
-        point(t.x - s.ix, t.y - s.iy, s.ig, a);
+        point(t.x - s.ix, t.y - s.iy, -s.g, a);
     }
     __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, Acc &a,
                                         bool self) {
@@ -418,7 +452,7 @@ struct DiskF64 {
         const double c = self ? 0.0 : s.g * rcp64(r2);
         a.u = __builtin_fma(-c, dy, a.u);
         a.v = __builtin_fma(c, dx, a.v);
-        point(t.x - s.ix, t.y - s.iy, s.ig, a);
+        point(t.x - s.ix, t.y - s.iy, -s.g, a);
     }
     // The reference's form (src/compute.cpp:318-329), for tiles holding a near-centre source.
     __device__ static void pair_ref_checked(const GeoParams &p, const Tgt &t, const Src &s,
@@ -460,25 +494,30 @@ struct DiskF64 {
         const double c = s.g * inv[0];
         a.u = __builtin_fma(-c, m.dy, a.u);
         a.v = __builtin_fma(c, m.dx, a.v);
-        const double k = s.ig * inv[1];
-        a.u = __builtin_fma(-k, m.ey, a.u);
-        a.v = __builtin_fma(k, m.ex, a.v);
+        const double k = s.g * inv[1];
+        a.u = __builtin_fma(k, m.ey, a.u);
+        a.v = __builtin_fma(-k, m.ex, a.v);
     }
     // Symmetric evaluation: the direct term is shared between (i, j) and (j, i); each side
-    // still gets its own image term (j's image at i, i's image at j).  The target carries its
-    // own image point.  Blocks holding a near-centre vortex use the reference's form both ways.
+    // still gets its own image term (j's image at i, i's image at j).  The two image distances
+    // share one reciprocal: |x_i - x_j*|^2 |x_j|^2 = |x_i|^2 |x_j|^2 - 2 R^2 x_i.x_j + R^4 is
+    // symmetric in i and j, so 1/|x_j - x_i*|^2 = |x_i|^2 w_j / |x_i - x_j*|^2.  Both
+    // differences are still formed directly (no cancellation); only the reciprocal is shared.
+    // Blocks holding a near-centre vortex use the reference's form both ways.
     static constexpr bool kSym = true;
-    static constexpr int kSymDenoms = 3;
+    static constexpr int kSymDenoms = 2;
+    using SymScalar = double;
+    using SymR = Acc;
     struct SymTgt {
-        double x, y, tx, ty, ix, iy;
+        double x, y, ix, iy, r2;
     };
     struct SymTmp {
-        double dx, dy, ex, ey, fx, fy;     // x_i - x_j, x_i - x_j*, x_j - x_i*
+        double dx, dy, ex, ey, fx, fy, rho;    // x_i - x_j, x_i - x_j*, x_j - x_i*, |x_i|^2 w_j
     };
     __device__ static double sym_g(const GeoParams &, double g) { return kInvTwoPi * g; }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) {
         const Src s = src(p, x, y, 0.0);
-        return {x, y, x * p.invR, y * p.invR, s.ix, s.iy};
+        return {x, y, s.ix, s.iy, __builtin_fma(x, x, y * y)};
     }
     __device__ static void sym_prep(const GeoParams &, const SymTgt &t, const Src &s, SymTmp &m,
                                     double *d) {
@@ -490,22 +529,22 @@ struct DiskF64 {
         d[1] = __builtin_fma(m.ex, m.ex, m.ey * m.ey);
         m.fx = s.x - t.ix;
         m.fy = s.y - t.iy;
-        d[2] = __builtin_fma(m.fx, m.fx, m.fy * m.fy);
+        m.rho = t.r2 * s.w;
     }
     __device__ static void sym_apply(const Src &s, double gi, const SymTmp &m, const double *inv,
                                      Acc &a, Acc &r) {
         const double c = s.g * inv[0];
         a.u = __builtin_fma(-c, m.dy, a.u);
         a.v = __builtin_fma(c, m.dx, a.v);
-        const double k = s.ig * inv[1];
-        a.u = __builtin_fma(-k, m.ey, a.u);
-        a.v = __builtin_fma(k, m.ex, a.v);
+        const double k = s.g * inv[1];             // j's image has strength -G_j
+        a.u = __builtin_fma(k, m.ey, a.u);
+        a.v = __builtin_fma(-k, m.ex, a.v);
         const double ci = gi * inv[0];
         r.u = __builtin_fma(ci, m.dy, r.u);
         r.v = __builtin_fma(-ci, m.dx, r.v);
-        const double ki = -gi * inv[2];            // i's image has strength -G_i
-        r.u = __builtin_fma(-ki, m.fy, r.u);
-        r.v = __builtin_fma(ki, m.fx, r.v);
+        const double ki = gi * (inv[1] * m.rho);   // i's image has strength -G_i
+        r.u = __builtin_fma(ki, m.fy, r.u);
+        r.v = __builtin_fma(-ki, m.fx, r.v);
     }
 };
 
@@ -532,7 +571,6 @@ struct DiskMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
-    static constexpr bool kSym = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         const double r2 = __builtin_fma(x, x, y * y);
         float ix = 1e30f, iy = 0.f;
@@ -581,6 +619,42 @@ struct DiskMixed {
     __device__ static void result(const Acc &a, double &u, double &v) {
         u = a.u.value();
         v = a.v.value();
+    }
+    // Symmetric evaluation: d = x_i - x_j once (FP64 difference, FP32 after).  j's image at i is
+    // d - delta_j, i's image at j is x_j - x_i* = -d - delta_i (the target carries delta_i); each
+    // side keeps its own FP32 reciprocal.  The reaction on the source is an FP32 sum (over at
+    // most 64 x 4 targets) before it joins the FP64 partials.
+    static constexpr bool kSym = true;
+    static constexpr int kSymDenoms = 0;
+    struct SymTgt {
+        double x, y;
+        float ix, iy;           // delta_i
+    };
+    struct SymTmp {};
+    using SymScalar = float;
+    struct SymR {
+        float u, v;
+    };
+    __device__ static float sym_g(const GeoParams &, double g) { return (float)(kInvTwoPi * g); }
+    __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) {
+        const Src s = src(p, x, y, 0.0);
+        return {x, y, s.ix, s.iy};
+    }
+    __device__ static void sym_pair(const GeoParams &, const SymTgt &t, float gi, const Src &s,
+                                    Acc &a, SymR &r) {
+        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+        const float inv = rcp32(__builtin_fmaf(dx, dx, dy * dy));
+        const float c = s.g * inv;
+        a.u32 = __builtin_fmaf(-c, dy, a.u32);
+        a.v32 = __builtin_fmaf(c, dx, a.v32);
+        const float ci = gi * inv;
+        r.u = __builtin_fmaf(ci, dy, r.u);
+        r.v = __builtin_fmaf(-ci, dx, r.v);
+        image(s, dx, dy, a);                       // -G_j at x_j*, on i
+        const float fx = -dx - t.ix, fy = -dy - t.iy;
+        const float k = gi * rcp32(__builtin_fmaf(fx, fx, fy * fy));
+        r.u = __builtin_fmaf(k, fy, r.u);          // -G_i at x_i*, on j
+        r.v = __builtin_fmaf(-k, fx, r.v);
     }
 };
 
@@ -678,6 +752,8 @@ struct PerF64 {
     static constexpr int kSymDenoms = 0;
     using SymTgt = Tgt;
     struct SymTmp {};
+    using SymScalar = double;
+    using SymR = Acc;
     __device__ static double sym_g(const GeoParams &p, double g) { return p.scale * g; }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
     __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, double gi, const Src &s,
@@ -714,7 +790,6 @@ struct PerMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
-    static constexpr bool kSym = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
     }
@@ -776,6 +851,27 @@ struct PerMixed {
         terms(p, t, s, fu, fv, self);
         a.u32 = __builtin_fmaf(-s.gs, fu, a.u32);
         a.v32 = __builtin_fmaf(s.gs, fv, a.v32);
+    }
+    // Symmetric evaluation as in PerF64 (the term is odd under X, Y -> -X, -Y); the reaction on
+    // the source is an FP32 sum (over at most 64 x 4 targets) before it joins the FP64 partials.
+    static constexpr bool kSym = true;
+    static constexpr int kSymDenoms = 0;
+    using SymTgt = Tgt;
+    struct SymTmp {};
+    using SymScalar = float;
+    struct SymR {
+        float u, v;
+    };
+    __device__ static float sym_g(const GeoParams &p, double g) { return (float)(p.scale * g); }
+    __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
+    __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, float gi, const Src &s,
+                                    Acc &a, SymR &r) {
+        float fu, fv;
+        terms(p, t, s, fu, fv, false);
+        a.u32 = __builtin_fmaf(-s.gs, fu, a.u32);
+        a.v32 = __builtin_fmaf(s.gs, fv, a.v32);
+        r.u = __builtin_fmaf(gi, fu, r.u);
+        r.v = __builtin_fmaf(-gi, fv, r.v);
     }
     __device__ static void fold(Acc &a) {
         a.u.add((double)a.u32);

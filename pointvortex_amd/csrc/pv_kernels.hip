@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
                         for (int t = 0; t < TPT; ++t)
                             P::prep(gp, tg[t], s2[q], m[q][t], &d[(q * TPT + t) * ND]);
                     const bool ok = TreeInverse<K>::run(d, inv);
-                    if (!__all(ok)) {
+                    if (tree_fallback(ok)) {
 #pragma unroll
                         for (int e = 0; e < K; ++e)
                             inv[e] = rcp64(d[e]);
@@ -208,6 +208,58 @@ template <class A> __device__ __forceinline__ void wave_sum(A &r) {
     }
 }
 
+// DPP wave_ror:1 (control 0x13C): every lane takes the value of its neighbour on one side, the
+// whole wave as one ring.  A double moves as two 32-bit halves.
+constexpr int kDppWaveRor1 = 0x13C;
+__device__ __forceinline__ double wave_ror1(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, kDppWaveRor1, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), kDppWaveRor1, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+__device__ __forceinline__ float wave_ror1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppWaveRor1, 0xF, 0xF, true));
+}
+
+// The ring's step in lane units (1 or 63, i.e. -1 mod 64): after k rotations lane l holds what
+// lane (l + k dir) mod 64 started with.  Read from the rotation itself, so the ring does not
+// depend on which way the hardware turns; the same for every lane.
+__device__ __forceinline__ int ring_dir(int lane) {
+    const int from = __builtin_amdgcn_mov_dpp(lane, kDppWaveRor1, 0xF, 0xF, true);
+    return __builtin_amdgcn_readfirstlane((from - lane) & 63);
+}
+
+// One source against a lane's targets: the target side into acc[t], the reaction on the source
+// into r.  The denominators of all TPT pairs share one reciprocal through the product tree.
+template <class P, int TPT>
+__device__ __forceinline__ void sym_step(const GeoParams &gp, const typename P::SymTgt *tg,
+                                         const typename P::SymScalar *gi,
+                                         const typename P::Src &s, typename P::Acc *acc,
+                                         typename P::SymR &r) {
+    if constexpr (P::kSymDenoms > 0) {
+        constexpr int ND = P::kSymDenoms;
+        typename P::SymTmp m[TPT];
+        double d[TPT * ND], inv[TPT * ND];
+#pragma unroll
+        for (int t = 0; t < TPT; ++t)
+            P::sym_prep(gp, tg[t], s, m[t], &d[t * ND]);
+        const bool ok = TreeInverse<TPT * ND>::run(d, inv);
+        if (tree_fallback(ok)) {
+#pragma unroll
+            for (int e = 0; e < TPT * ND; ++e)
+                inv[e] = rcp64(d[e]);
+        }
+#pragma unroll
+        for (int t = 0; t < TPT; ++t)
+            P::sym_apply(s, gi[t], m[t], &inv[t * ND], acc[t], r);
+    } else {
+#pragma unroll
+        for (int t = 0; t < TPT; ++t)
+            P::sym_pair(gp, tg[t], gi[t], s, acc[t], r);
+    }
+}
+
 // Block -> tile pair (I <= J) in row-major triangular order: b = J (J + 1) / 2 + I.
 __device__ __forceinline__ void tile_pair(int b, int &I, int &J) {
     int j = (int)((__builtin_sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
@@ -234,7 +286,8 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
     const int s0 = J * TT, s1 = min(a.n, s0 + TT);
     const size_t n = (size_t)a.n;
 
-    double xi[TPT], yi[TPT], gi[TPT];
+    double xi[TPT], yi[TPT];
+    typename P::SymScalar gi[TPT];
     int ti[TPT];
     int near = 0;
 #pragma unroll
@@ -244,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
         const int ii = valid ? i : t0;
         xi[t] = a.x[ii];
         yi[t] = a.y[ii];
-        gi[t] = valid ? P::sym_g(gp, a.g[ii]) : 0.0;     // a padding target adds no reaction
+        gi[t] = valid ? P::sym_g(gp, a.g[ii]) : 0;       // a padding target adds no reaction
         ti[t] = valid ? i : -1;
         if constexpr (P::kNearTiles)
             near |= valid && P::is_near(gp, xi[t], yi[t]);
@@ -286,17 +339,25 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
                     }
                 }
             }
+            if constexpr (P::kFoldEvery > 0) {
+#pragma unroll
+                for (int t = 0; t < TPT; ++t)
+                    P::fold(acc[t]);
+            }
         }
     } else {
         typename P::SymTgt tg[TPT];
 #pragma unroll
         for (int t = 0; t < TPT; ++t)
             tg[t] = P::sym_tgt(gp, xi[t], yi[t]);
+        const int dir = ring_dir(lane);
         for (int base = s0; base < s1; base += TILE) {
             const int cnt = min(TILE, s1 - base);
+            const int span = (cnt + 63) & ~63;          // <= TILE: TILE is a multiple of 64
             __syncthreads();
-            for (int k = tid; k < cnt; k += kBlock)
-                sh[k] = P::src(gp, a.x[base + k], a.y[base + k], a.g[base + k]);
+            for (int k = tid; k < span; k += kBlock)
+                sh[k] = k < cnt ? P::src(gp, a.x[base + k], a.y[base + k], a.g[base + k])
+                                : P::src(gp, a.x[base], a.y[base], 0.0);
             __syncthreads();
             bool done = false;
             if constexpr (P::kNearTiles) {
@@ -329,75 +390,36 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
                 }
             }
             if (!done) {
-                if constexpr (P::kSymDenoms > 0) {
-                    constexpr int ND = P::kSymDenoms;
-                    int k = 0;
-                    for (; k + 2 <= cnt; k += 2) {
-                        const typename P::Src s2[2] = {sh[k], sh[k + 1]};
-                        typename P::SymTmp m[2][TPT];
-                        double d[2 * TPT * ND], inv[2 * TPT * ND];
-#pragma unroll
-                        for (int q = 0; q < 2; ++q)
-#pragma unroll
-                            for (int t = 0; t < TPT; ++t)
-                                P::sym_prep(gp, tg[t], s2[q], m[q][t], &d[(q * TPT + t) * ND]);
-                        const bool ok = TreeInverse<2 * TPT * ND>::run(d, inv);
-                        if (!__all(ok)) {
-#pragma unroll
-                            for (int e = 0; e < 2 * TPT * ND; ++e)
-                                inv[e] = rcp64(d[e]);
-                        }
-#pragma unroll
-                        for (int q = 0; q < 2; ++q) {
-                            Acc r;
-                            P::zero(r);
-#pragma unroll
-                            for (int t = 0; t < TPT; ++t)
-                                P::sym_apply(s2[q], gi[t], m[q][t], &inv[(q * TPT + t) * ND],
-                                             acc[t], r);
-                            wave_sum(r);
-                            if (lane == 0) {
-                                red[wave][k + q][0] = r.u;
-                                red[wave][k + q][1] = r.v;
-                            }
-                        }
+                // Ring: lane l starts on source g0 + l with that source's reaction in r; after
+                // each step r moves one lane (wave_ror:1) and the lane reads the source whose
+                // reaction it now holds.  64 steps bring every reaction back to its first lane
+                // with the sum over the wave's targets.  Staged pads (strength 0) fill the last
+                // group of 64; their reactions are dropped.
+                for (int g0 = 0; g0 < cnt; g0 += 64) {
+                    typename P::SymR r;
+                    r.u = 0;
+                    r.v = 0;
+                    typename P::Src next = sh[g0 + lane];
+#pragma unroll 2
+                    for (int s = 0; s < 64; ++s) {
+                        // The next step's source is read one step ahead (at s = 63 this reads
+                        // the lane's first source again, unused).
+                        const typename P::Src s_a = next;
+                        next = sh[g0 + ((lane + dir * (s + 1)) & 63)];
+                        sym_step<P, TPT>(gp, tg, gi, s_a, acc, r);
+                        r.u = wave_ror1(r.u);
+                        r.v = wave_ror1(r.v);
                     }
-                    if (k < cnt) {
-                        const typename P::Src s_a = sh[k];
-                        typename P::SymTmp m[TPT];
-                        double d[TPT * ND], inv[TPT * ND];
-#pragma unroll
-                        for (int t = 0; t < TPT; ++t)
-                            P::sym_prep(gp, tg[t], s_a, m[t], &d[t * ND]);
-#pragma unroll
-                        for (int e = 0; e < TPT * ND; ++e)
-                            inv[e] = rcp64(d[e]);
-                        Acc r;
-                        P::zero(r);
-#pragma unroll
-                        for (int t = 0; t < TPT; ++t)
-                            P::sym_apply(s_a, gi[t], m[t], &inv[t * ND], acc[t], r);
-                        wave_sum(r);
-                        if (lane == 0) {
-                            red[wave][k][0] = r.u;
-                            red[wave][k][1] = r.v;
-                        }
-                    }
-                } else {
-                    for (int k = 0; k < cnt; ++k) {
-                        const typename P::Src s_a = sh[k];
-                        Acc r;
-                        P::zero(r);
-#pragma unroll
-                        for (int t = 0; t < TPT; ++t)
-                            P::sym_pair(gp, tg[t], gi[t], s_a, acc[t], r);
-                        wave_sum(r);
-                        if (lane == 0) {
-                            red[wave][k][0] = r.u;
-                            red[wave][k][1] = r.v;
-                        }
+                    if (g0 + lane < cnt) {
+                        red[wave][g0 + lane][0] = (double)r.u;
+                        red[wave][g0 + lane][1] = (double)r.v;
                     }
                 }
+            }
+            if constexpr (P::kFoldEvery > 0) {
+#pragma unroll
+                for (int t = 0; t < TPT; ++t)
+                    P::fold(acc[t]);
             }
             __syncthreads();
             // The reaction on this chunk's sources: the four waves in order, into slot I.
@@ -425,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
     }
 }
 
-template <class P>
+// Partials are FP64 in both precisions, so one reduction serves every policy.
 __global__ __launch_bounds__(kBlock) void sym_reduce_kernel(SymLaunch a, GeoParams gp) {
     if (a.st->fail_epoch < a.epoch)
         return;
@@ -784,8 +806,7 @@ void launch_pair(int geometry, int precision, int tpt, int tile, const PairLaunc
 }
 
 bool sym_supported(int geometry, int precision) {
-    (void)geometry;
-    return precision == 0;
+    return geometry >= 0 && geometry <= 2 && (precision == 0 || precision == 1);
 }
 
 template <class P, int TILE>
@@ -808,24 +829,28 @@ static void launch_sym_p(int tpt, int tile, const SymLaunch &a, const GeoParams 
         launch_sym_tpt<P, 256>(tpt, a, gp, s);
 }
 
-void launch_sym(int geometry, int tpt, int tile, const SymLaunch &a, const GeoParams &gp,
-                hipStream_t s) {
-    if (geometry == 0)
-        launch_sym_p<InfF64>(tpt, tile, a, gp, s);
-    else if (geometry == 1)
-        launch_sym_p<PerF64>(tpt, tile, a, gp, s);
-    else
-        launch_sym_p<DiskF64>(tpt, tile, a, gp, s);
+void launch_sym(int geometry, int precision, int tpt, int tile, const SymLaunch &a,
+                const GeoParams &gp, hipStream_t s) {
+    if (precision == 0) {
+        if (geometry == 0)
+            launch_sym_p<InfF64>(tpt, tile, a, gp, s);
+        else if (geometry == 1)
+            launch_sym_p<PerF64>(tpt, tile, a, gp, s);
+        else
+            launch_sym_p<DiskF64>(tpt, tile, a, gp, s);
+    } else {
+        if (geometry == 0)
+            launch_sym_p<InfMixed>(tpt, tile, a, gp, s);
+        else if (geometry == 1)
+            launch_sym_p<PerMixed>(tpt, tile, a, gp, s);
+        else
+            launch_sym_p<DiskMixed>(tpt, tile, a, gp, s);
+    }
 }
 
-void launch_sym_reduce(int geometry, const SymLaunch &a, const GeoParams &gp, hipStream_t s) {
+void launch_sym_reduce(const SymLaunch &a, const GeoParams &gp, hipStream_t s) {
     const dim3 grid((unsigned)((a.n + kBlock - 1) / kBlock));
-    if (geometry == 0)
-        hipLaunchKernelGGL((sym_reduce_kernel<InfF64>), grid, dim3(kBlock), 0, s, a, gp);
-    else if (geometry == 1)
-        hipLaunchKernelGGL((sym_reduce_kernel<PerF64>), grid, dim3(kBlock), 0, s, a, gp);
-    else
-        hipLaunchKernelGGL((sym_reduce_kernel<DiskF64>), grid, dim3(kBlock), 0, s, a, gp);
+    hipLaunchKernelGGL(sym_reduce_kernel, grid, dim3(kBlock), 0, s, a, gp);
 }
 
 void launch_make_stage(const StageOp &op, int n, const DevStatus *st, unsigned epoch,

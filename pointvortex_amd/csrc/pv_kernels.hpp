@@ -77,11 +77,12 @@ constexpr int kTileMixed = 128;
 void launch_pair(int geometry, int precision, int tpt, int tile, const PairLaunch &a,
                  const GeoParams &gp, hipStream_t s);
 int pair_tile(int precision);
-// Symmetric path: supported for the FP64 policies; two launches (pairs, then reduce+epilogue).
+// Symmetric path (every geometry and precision): two launches, the pairs then the
+// reduce + epilogue.
 bool sym_supported(int geometry, int precision);
-void launch_sym(int geometry, int tpt, int tile, const SymLaunch &a, const GeoParams &gp,
-                hipStream_t s);
-void launch_sym_reduce(int geometry, const SymLaunch &a, const GeoParams &gp, hipStream_t s);
+void launch_sym(int geometry, int precision, int tpt, int tile, const SymLaunch &a,
+                const GeoParams &gp, hipStream_t s);
+void launch_sym_reduce(const SymLaunch &a, const GeoParams &gp, hipStream_t s);
 void launch_make_stage(const StageOp &op, int n, const DevStatus *st, unsigned epoch,
                        hipStream_t s);
 void launch_reset_status(DevStatus *st, hipStream_t s);

@@ -120,6 +120,41 @@ def test_symmetric_path_keeps_parity(oracle, geometry, n, tpt, tile):
                                                        ur, vr)
 
 
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+@pytest.mark.parametrize("n", [2, 700, 5000])
+@pytest.mark.parametrize("tpt,tile", [(1, 128), (2, 256), (4, 128)])
+def test_symmetric_mixed_keeps_parity(oracle, geometry, n, tpt, tile):
+    """Mixed precision over unordered pairs: within 1e-6 of the oracle, and no less accurate
+    than the split path's mixed sums (the reaction is an FP32 sum of at most 64 x 4 terms, the
+    same class as the split path's per-tile FP32 sums)."""
+    pop, k = setup(geometry, "mixed", n, seed=9 + n)
+    s = VortexSystem(pop.x, pop.y, pop.circulation)
+    k.set_launch(tpt, 0, tile, 1)
+    assert k.resolved_launch(pop.n, pop.n) == (tpt, 0, tile, 1)
+    vel = k.evaluate(s)
+    k.set_launch(tpt, 4, tile, 0)
+    split = k.evaluate(s)
+    ur, vr = oracle.evaluate(spec(geometry, box_length=2.0), pop.x, pop.y, pop.circulation)
+    err, err_split = normwise(vel.x, vel.y, ur, vr), normwise(split.x, split.y, ur, vr)
+    assert err <= 1e-6
+    assert err <= max(3.0 * err_split, 1e-7), (err, err_split)
+
+
+@pytest.mark.parametrize("pos", [(2e-160, 1e-161), (-3e-31, 2e-31)])
+def test_symmetric_mixed_near_centre_disk_source(oracle, pos):
+    """A near-centre vortex (its image a far, finite offset) in a middle tile, so that it is a
+    source for earlier tiles and a target for later ones.  One such vortex only: two of them
+    would sit closer than FP32 can square."""
+    pop, k = setup("disk", "mixed", 1800, seed=73)
+    x, y = pop.x.copy(), pop.y.copy()
+    x[900], y[900] = pos
+    k.set_launch(2, 0, 128, 1)
+    vel = k.evaluate(VortexSystem(x, y, pop.circulation))
+    assert np.all(np.isfinite(vel.x)) and np.all(np.isfinite(vel.y))
+    ur, vr = oracle.evaluate(spec("disk"), x, y, pop.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= 1e-6
+
+
 def test_symmetric_path_near_centre_disk_source(oracle):
     pop, k = setup("disk", "fp64", 1800, seed=71)
     x, y = pop.x.copy(), pop.y.copy()
@@ -132,8 +167,8 @@ def test_symmetric_path_near_centre_disk_source(oracle):
 
 @pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
 def test_symmetric_path_in_rk4_and_fallbacks(oracle, geometry):
-    """RK4 steps through the symmetric path agree with the split path; mixed precision and
-    target subsets fall back to the split path; repeats are bit-identical."""
+    """RK4 steps through the symmetric path agree with the split path; target subsets fall
+    back to the split path; repeats are bit-identical."""
     pop, ks = setup(geometry, "fp64", 3000, seed=12)
     _, kp = setup(geometry, "fp64", 3000, seed=12)
     ks.set_launch(2, 0, 256, 1)
@@ -158,4 +193,5 @@ def test_symmetric_path_in_rk4_and_fallbacks(oracle, geometry):
     assert normwise(sub.x[100:400], sub.y[100:400], ur[100:400], vr[100:400]) <= 1e-12
     _, km = setup(geometry, "mixed", 3000, seed=12)
     km.set_launch(2, 0, 256, 1)
-    assert km.resolved_launch(3000, 3000)[3] == 0
+    assert km.resolved_launch(3000, 3000)[3] == 1
+    assert km.resolved_launch(100, 3000)[3] == 0
