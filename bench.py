@@ -197,7 +197,10 @@ def ensemble_bench(pv, steps, count=4096):
                 reasons[key] = reasons.get(key, 0) + 1
         out[prec] = {"wall_s": wall, "solves_per_s": solved / wall, "pairs_per_s": pairs / wall,
                      "completed_instances": solved, "failed_instances": count - solved,
-                     "failures": reasons}
+                     "failures": reasons,
+                     "decomposition": ("symmetric: pairs of 128-vortex tiles, each unordered "
+                                       "pair once" if e.symmetric else
+                                       "ordered: 64-target items against all sources")}
         log(f"[bench] ensemble {prec}: {wall:.2f} s, {solved / wall:.1f} solves/s, "
             f"{pairs / wall:.3e} pairs/s, failed {count - solved}: {reasons}")
         e.close()

@@ -234,6 +234,18 @@ void pv_batch_destroy(pv_batch *b);
 pv_status pv_batch_set_precision(pv_batch *b, int precision);
 size_t pv_batch_bytes_per_vortex(void);
 
+/* How an ensemble evaluates its pairs (no reference counterpart; the reference runs one
+ * system per process).  mode 1: symmetric -- work items are pairs of 128-vortex tiles of one
+ * instance, each unordered pair computed once, with per-instance partial sums (16 B per
+ * vortex per 128 vortices of its instance, on top of the 208 B per vortex) reduced in a fixed
+ * order; mode 0: ordered -- 64-target items against all of the instance's sources; mode -1
+ * (default): symmetric for fixed-step ensembles (pv_batch_load), ordered for admitted runs,
+ * whose admission budget counts 208 B per vortex.  Both give every instance a result that
+ * does not depend on the rest of the ensemble.  pv_batch_symmetric reports the mode the
+ * current work items use. */
+pv_status pv_batch_set_symmetric(pv_batch *b, int mode);
+int pv_batch_symmetric(const pv_batch *b);
+
 /* Fixed-step ensembles: replace the contents with `ninst` instances laid out one after the
  * other in x, y, gamma (counts[k] vortices each), advance them with RK4 (optionally as
  * replays of a captured hipGraph), read positions back, or evaluate velocities.

@@ -163,6 +163,8 @@ _SIGNATURES = {
                                        ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "pv_batch_destroy": (None, [_vp]),
     "pv_batch_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pv_batch_set_symmetric": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pv_batch_symmetric": (ctypes.c_int, [_vp]),
     "pv_batch_bytes_per_vortex": (ctypes.c_size_t, []),
     "pv_batch_load": (ctypes.c_int, [_vp, ctypes.POINTER(PvInstanceConfig), _szp,
                                      ctypes.c_size_t, _dp, _dp, _dp]),
@@ -491,5 +493,5 @@ def declared_symbols(header: Path = HEADER_PATH):
     import re
 
     text = header.read_text()
-    return sorted(set(re.findall(r"^\s*(?:pv_status|void|const char \*|size_t)\s*(pv_\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:pv_status|void|const char \*|size_t|int)\s*(pv_\w+)\s*\(",
                                  text, re.M)))

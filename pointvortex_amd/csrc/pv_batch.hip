@@ -15,6 +15,7 @@
 //             applies src/timestep.cpp:140-153 to every instance on the device.
 #include "pv_batch.hpp"
 #include "pv_epilogue.hpp"
+#include "pv_sym.hpp"
 
 namespace pv {
 
@@ -158,6 +159,258 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(BatchLaunch a) {
     }
 }
 
+// ------------------------------------------------------------------ symmetric mode
+
+// Per-wave LDS of the symmetric item: the kSymTile source records of tile J, and the reaction
+// sums of the near-centre disk path (written by every lane with the same value, read back one
+// source per lane, so no lane-0-only store sits inside the persistent loop).
+template <class P> struct SymSlice {
+    typename P::Src src[kSymTile];
+    double red[kSymTile][2];
+};
+
+// One pair of tiles (I, J) of one instance: partial slot J of tile I's targets, and for I < J
+// slot I of tile J's sources (the reaction).  I == J: ordered pairs inside the tile.
+template <class P>
+__device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const PairItem &it,
+                               SymSlice<P> *sl, int lane, int dir) {
+    using Acc = typename P::Acc;
+    constexpr int TPT = kSymTpt, TT = kSymTile;
+    const GeoParams &gp = ins->gp;
+    const int off = ins->offset, n = ins->n;
+    const size_t nn = (size_t)n;
+    double *part = a.part + ins->poff;
+    const int t0 = it.I * TT, s0 = it.J * TT;
+    const int cnt = min(TT, n - s0);
+    const int span = (cnt + 63) & ~63;
+
+    double xi[TPT], yi[TPT];
+    typename P::SymScalar gi[TPT];
+    int ti[TPT];
+    bool near = false;
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        const int i = t0 + t * 64 + lane;
+        const bool valid = i < n;
+        const int ii = off + (valid ? i : t0);
+        xi[t] = a.x[ii];
+        yi[t] = a.y[ii];
+        gi[t] = valid ? P::sym_g(gp, a.g[ii]) : 0;     // a padding target adds no reaction
+        ti[t] = valid ? i : -1;
+        if constexpr (P::kNearTiles)
+            near |= valid && P::is_near(gp, xi[t], yi[t]);
+    }
+    wave_sync();
+    for (int k = lane; k < span; k += 64) {
+        const int j = off + s0 + (k < cnt ? k : 0);
+        const double xj = a.x[j], yj = a.y[j];
+        sl->src[k] = P::src(gp, xj, yj, k < cnt ? a.g[j] : 0.0);
+        if constexpr (P::kNearTiles)
+            near |= k < cnt && P::is_near(gp, xj, yj);
+    }
+    wave_sync();
+    if constexpr (P::kNearTiles)
+        near = __any(near);
+
+    Acc acc[TPT];
+#pragma unroll
+    for (int t = 0; t < TPT; ++t)
+        P::zero(acc[t]);
+
+    if (it.I == it.J) {
+        typename P::Tgt tg[TPT];
+#pragma unroll
+        for (int t = 0; t < TPT; ++t)
+            tg[t] = P::tgt(gp, xi[t], yi[t]);
+        for (int k = 0; k < cnt; ++k) {
+            const typename P::Src s_a = sl->src[k];
+#pragma unroll
+            for (int t = 0; t < TPT; ++t) {
+                if constexpr (P::kNearTiles) {
+                    if (near)
+                        P::pair_ref_checked(gp, tg[t], s_a, acc[t], ti[t] == s0 + k);
+                    else
+                        P::pair_checked(gp, tg[t], s_a, acc[t], ti[t] == s0 + k);
+                } else {
+                    P::pair_checked(gp, tg[t], s_a, acc[t], ti[t] == s0 + k);
+                }
+            }
+            if constexpr (P::kFoldEvery > 0) {
+                if ((k & 63) == 63 || k == cnt - 1) {       // result() reads folded sums only
+#pragma unroll
+                    for (int t = 0; t < TPT; ++t)
+                        P::fold(acc[t]);
+                }
+            }
+        }
+    } else {
+        bool done = false;
+        if constexpr (P::kNearTiles) {
+            if (near) {
+                // The reference's form both ways: j on i, and i (as a source) on j.
+                typename P::Tgt tr[TPT];
+                typename P::Src si[TPT];
+#pragma unroll
+                for (int t = 0; t < TPT; ++t) {
+                    tr[t] = P::tgt(gp, xi[t], yi[t]);
+                    si[t] = P::src(gp, xi[t], yi[t], ti[t] >= 0 ? a.g[off + ti[t]] : 0.0);
+                }
+                for (int k = 0; k < cnt; ++k) {
+                    const typename P::Src s_a = sl->src[k];
+                    const typename P::Tgt tj = P::tgt(gp, s_a.x, s_a.y);
+                    Acc r;
+                    P::zero(r);
+#pragma unroll
+                    for (int t = 0; t < TPT; ++t) {
+                        P::pair_ref_checked(gp, tr[t], s_a, acc[t], false);
+                        P::pair_ref_checked(gp, tj, si[t], r, false);
+                    }
+                    wave_sum(r);
+                    sl->red[k][0] = r.u;
+                    sl->red[k][1] = r.v;
+                }
+                wave_sync();
+                for (int k = lane; k < cnt; k += 64) {
+                    part[(size_t)(2 * it.I) * nn + s0 + k] = sl->red[k][0];
+                    part[(size_t)(2 * it.I + 1) * nn + s0 + k] = sl->red[k][1];
+                }
+                done = true;
+            }
+        }
+        if (!done) {
+            typename P::SymTgt tg[TPT];
+#pragma unroll
+            for (int t = 0; t < TPT; ++t)
+                tg[t] = P::sym_tgt(gp, xi[t], yi[t]);
+            // The ring of the single-system symmetric kernel: 64 steps per group of 64
+            // sources bring every source's reaction back to its first lane.
+            for (int g0 = 0; g0 < span; g0 += 64) {
+                typename P::SymR r;
+                r.u = 0;
+                r.v = 0;
+                typename P::Src next = sl->src[g0 + lane];
+#pragma unroll 2
+                for (int s = 0; s < 64; ++s) {
+                    const typename P::Src s_a = next;
+                    next = sl->src[g0 + ((lane + dir * (s + 1)) & 63)];
+                    sym_step<P, TPT>(gp, tg, gi, s_a, acc, r);
+                    r.u = wave_ror1(r.u);
+                    r.v = wave_ror1(r.v);
+                }
+                if (g0 + lane < cnt) {
+                    part[(size_t)(2 * it.I) * nn + s0 + g0 + lane] = (double)r.u;
+                    part[(size_t)(2 * it.I + 1) * nn + s0 + g0 + lane] = (double)r.v;
+                }
+                if constexpr (P::kFoldEvery > 0) {
+#pragma unroll
+                    for (int t = 0; t < TPT; ++t)
+                        P::fold(acc[t]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TPT; ++t) {
+        double u, v;
+        P::result(acc[t], u, v);
+        if (ti[t] >= 0) {
+            part[(size_t)(2 * it.J) * nn + ti[t]] = u;
+            part[(size_t)(2 * it.J + 1) * nn + ti[t]] = v;
+        }
+    }
+}
+
+template <class PI, class PP, class PD, bool ADAPT>
+__global__ __launch_bounds__(kBatchBlock) void batch_sym_kernel(BatchLaunch a) {
+    constexpr size_t kSliceBytes =
+        sizeof(SymSlice<PD>) > sizeof(SymSlice<PI>)
+            ? (sizeof(SymSlice<PD>) > sizeof(SymSlice<PP>) ? sizeof(SymSlice<PD>)
+                                                           : sizeof(SymSlice<PP>))
+            : (sizeof(SymSlice<PI>) > sizeof(SymSlice<PP>) ? sizeof(SymSlice<PI>)
+                                                           : sizeof(SymSlice<PP>));
+    __shared__ __align__(16) unsigned char lds[kBatchBlock / 64][kSliceBytes];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned char *mine = lds[wave];
+    const int dir = ring_dir(lane);
+    for (;;) {
+        // One chunk of consecutive items per atomic: there are about 4x as many tile-pair
+        // items as ordered ones, and one device-scope atomic per item would serialise on the
+        // counter.
+        int chunk = 0;
+        if (lane == 0)
+            chunk = (int)atomicAdd(a.counter, 1u);
+        chunk = __builtin_amdgcn_readfirstlane(__shfl(chunk, 0, 64));
+        if (chunk >= a.nchunks)
+            break;
+        const int first = __builtin_amdgcn_readfirstlane(a.chunks[chunk]);
+        const int last = __builtin_amdgcn_readfirstlane(a.chunks[chunk + 1]);
+        for (int item = first; item < last; ++item) {
+            const PairItem it = a.pitems[item];
+            const Instance *ins = &a.inst[it.instance];
+            // The item-skip rules of batch_kernel; the reduction applies them per vortex.
+            const InstanceStep &sp = a.steps[it.instance];
+            if (sp.failure)
+                continue;
+            if (ADAPT && (sp.state != kRunning || (a.pass == kPassNeedK0 && sp.fsal)))
+                continue;
+            const int geometry = ins->gp.geometry;
+            if (geometry == 0)
+                batch_sym_item<PI>(a, ins, it, reinterpret_cast<SymSlice<PI> *>(mine), lane, dir);
+            else if (geometry == 1)
+                batch_sym_item<PP>(a, ins, it, reinterpret_cast<SymSlice<PP> *>(mine), lane, dir);
+            else
+                batch_sym_item<PD>(a, ins, it, reinterpret_cast<SymSlice<PD> *>(mine), lane, dir);
+        }
+    }
+}
+
+// Per flat vortex: the validation and the slot-ordered compensated sum of its instance's
+// partials, then the fused stage epilogue (as batch_item's tail).
+template <bool ADAPT>
+__global__ __launch_bounds__(kBatchBlock) void batch_sym_reduce_kernel(BatchLaunch a) {
+    const int v = blockIdx.x * kBatchBlock + threadIdx.x;
+    if (v >= a.span)
+        return;
+    const int k = a.owner[v];
+    if (k < 0)
+        return;
+    InstanceStep *stp = &a.steps[k];
+    if (stp->failure)
+        return;
+    if (ADAPT && (stp->state != kRunning || (a.pass == kPassNeedK0 && stp->fsal)))
+        return;
+    const Instance ins = a.inst[k];
+    FailSink fail{a.st, a.epoch, &stp->failure};
+    if (!ADAPT) {
+        fail.keep = stp->keep;
+        fail.x0 = a.op.x0;
+        fail.y0 = a.op.y0;
+        fail.steps_at_failure = &stp->fail_step;
+        fail.steps_done = a.counter + 1;
+    }
+    const double xt = a.x[v], yt = a.y[v];
+    if (!finite64(xt) || !finite64(yt))
+        fail(kFailStateNonFinite);
+    else if (ins.gp.geometry == 2 && xt * xt + yt * yt >= ins.gp.R2)
+        fail(kFailOutsideDisk);
+    const size_t nn = (size_t)ins.n;
+    const double *p = a.part + ins.poff + (v - ins.offset);
+    Comp su, sv;
+    su.zero();
+    sv.zero();
+    for (int s = 0; s < ins.nt; ++s) {
+        su.add(__builtin_nontemporal_load(p + (size_t)(2 * s) * nn));
+        sv.add(__builtin_nontemporal_load(p + (size_t)(2 * s + 1) * nn));
+    }
+    const double dt = ADAPT ? stp->dt : a.op.dt;
+    const double atol = ADAPT ? stp->atol : a.op.atol;
+    const double rtol = ADAPT ? stp->rtol : a.op.rtol;
+    const double err =
+        stage_epilogue(a.op, dt, atol, rtol, fail, a.x, a.y, v, su.value(), sv.value());
+    if (ADAPT && a.op.kind == kOpDopriError)
+        atomicMax(&stp->err_bits, (unsigned long long)__double_as_longlong(err));
+}
+
 __global__ void batch_reset_kernel(unsigned *counter) {
     if (threadIdx.x == 0)
         *counter = 0u;
@@ -265,6 +518,34 @@ void launch_batch(int precision, bool adaptive, const BatchLaunch &a, int blocks
             hipLaunchKernelGGL((batch_kernel<InfMixed, PerMixed, DiskMixed, false>), grid, block, 0,
                                s, a);
     }
+}
+
+void launch_batch_sym(int precision, bool adaptive, const BatchLaunch &a, int blocks,
+                      hipStream_t s) {
+    const dim3 grid(blocks), block(kBatchBlock);
+    if (precision == 0) {
+        if (adaptive)
+            hipLaunchKernelGGL((batch_sym_kernel<InfF64, PerF64, DiskF64, true>), grid, block, 0,
+                               s, a);
+        else
+            hipLaunchKernelGGL((batch_sym_kernel<InfF64, PerF64, DiskF64, false>), grid, block, 0,
+                               s, a);
+    } else {
+        if (adaptive)
+            hipLaunchKernelGGL((batch_sym_kernel<InfMixed, PerMixed, DiskMixed, true>), grid,
+                               block, 0, s, a);
+        else
+            hipLaunchKernelGGL((batch_sym_kernel<InfMixed, PerMixed, DiskMixed, false>), grid,
+                               block, 0, s, a);
+    }
+}
+
+void launch_batch_sym_reduce(bool adaptive, const BatchLaunch &a, hipStream_t s) {
+    const dim3 grid((unsigned)((a.span + kBatchBlock - 1) / kBatchBlock)), block(kBatchBlock);
+    if (adaptive)
+        hipLaunchKernelGGL(batch_sym_reduce_kernel<true>, grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL(batch_sym_reduce_kernel<false>, grid, block, 0, s, a);
 }
 
 void launch_batch_reset(unsigned *counter, hipStream_t s) {

@@ -6,10 +6,14 @@
 namespace pv {
 
 // One instance of the ensemble: its vortices are [offset, offset + n) of the flat SoA arrays.
+// Symmetric mode: the instance's partial sums are nt tile rows x 2n doubles at part + poff.
 struct Instance {
     GeoParams gp;
     int offset;
     int n;
+    int nt;
+    int pad;
+    size_t poff;
 };
 
 // One unit of work: 64 targets [tbegin, tend) (instance-local) of one instance against all
@@ -40,6 +44,21 @@ struct InstanceStep {
     const double *keep[2];            // fixed-step mode: {x, y} before the failing step
 };
 
+// Symmetric mode: one unit of work is a pair of kSymTile-vortex tiles (I <= J) of one
+// instance.  One wave computes each of its unordered pairs once and writes the reaction on
+// tile J into partial slot I and tile I's side into slot J; every (slot, vortex) is written
+// by exactly one item, and a reduction adds a vortex's slots in slot order.
+constexpr int kSymTpt = 2;
+constexpr int kSymTile = 64 * kSymTpt;
+// A wave takes one chunk of consecutive items per counter increment: the host cuts the
+// cost-sorted items into runs of about equal cost (the largest items alone), so the counter
+// sees far fewer atomics than items and the tail stays balanced.
+struct PairItem {
+    int instance;
+    int I, J;
+    int pad;
+};
+
 enum InstanceState : int { kRunning = 0, kFinished = 1, kFailed = 2, kIdle = 3 };
 constexpr unsigned kFailRejected = 32u;
 
@@ -60,11 +79,23 @@ struct BatchLaunch {
     StageOp op;
     InstanceStep *steps;       // adaptive mode: per-instance dt, error and failure sink
     int pass;
+    // symmetric mode
+    const PairItem *pitems;    // work items (nitems of them) instead of `items`
+    const int *chunks;         // chunk c is items [chunks[c], chunks[c + 1]); nchunks chunks
+    int nchunks;
+    double *part;              // partial sums, Instance::poff per instance
+    const int *owner;          // instance of every flat vortex (-1: unused)
+    int span;                  // flat vortices the reduction covers
 };
 
 // Launchers (pv_batch.hip).
 void launch_batch(int precision, bool adaptive, const BatchLaunch &a, int blocks, hipStream_t s);
 void launch_batch_reset(unsigned *counter, hipStream_t s);
+// Symmetric mode: the pair kernel over PairItems, then the per-vortex reduction with the
+// stage epilogue.
+void launch_batch_sym(int precision, bool adaptive, const BatchLaunch &a, int blocks,
+                      hipStream_t s);
+void launch_batch_sym_reduce(bool adaptive, const BatchLaunch &a, hipStream_t s);
 void launch_batch_step_done(unsigned *counter, hipStream_t s);
 void launch_batch_stage1(const StageOp &op, const int *owner, const InstanceStep *steps, int n,
                          hipStream_t s);

@@ -103,6 +103,17 @@ struct pv_batch {
     pv::WorkItem *items_d = nullptr;
     size_t items_cap = 0;
     int nitems = 0;
+    // symmetric mode (pv::PairItem work items, per-instance partial sums)
+    int sym_mode = -1;                 // -1: fixed-step ensembles only; 0: never; 1: always
+    bool sym = false;                  // the mode the current work items were built for
+    pv::PairItem *pitems_d = nullptr;
+    size_t pitems_cap = 0;
+    int *chunks_d = nullptr;           // pitems_cap + 1 chunk starts
+    int nchunks = 0;
+    double *part = nullptr;
+    size_t part_cap = 0;               // doubles
+    int span = 0;                      // flat vortices the reduction covers
+    std::vector<int> item_instances;   // instances of the current work items
     unsigned *counter = nullptr;
     pv::DevStatus *st = nullptr;
     pv::DevStatus *st_host = nullptr;
@@ -132,6 +143,7 @@ struct pv_batch {
             dfree(kx[s]), dfree(ky[s]);
         dfree(u), dfree(v), dfree(owner);
         dfree(inst_d), dfree(steps_d), dfree(items_d), dfree(counter), dfree(st);
+        dfree(pitems_d), dfree(chunks_d), dfree(part);
         if (steps_h)
             (void)hipHostFree(steps_h);
         if (st_host)
@@ -174,12 +186,116 @@ void launch(pv_batch *b, const double *px, const double *py, const pv::StageOp &
     a.op = op;
     a.steps = b->steps_d;
     a.pass = pass;
+    if (b->sym) {
+        a.pitems = b->pitems_d;
+        a.chunks = b->chunks_d;
+        a.nchunks = b->nchunks;
+        a.part = b->part;
+        a.owner = b->owner;
+        a.span = b->span;
+        pv::launch_batch_sym(b->precision, adaptive, a, batch_blocks(b), b->stream);
+        hip_check(hipGetLastError(), "launch HIP ensemble pair kernel");
+        pv::launch_batch_sym_reduce(adaptive, a, b->stream);
+        hip_check(hipGetLastError(), "launch HIP ensemble reduction");
+        return;
+    }
     pv::launch_batch(b->precision, adaptive, a, batch_blocks(b), b->stream);
     hip_check(hipGetLastError(), "launch HIP ensemble kernel");
 }
 
-// Cost-sorted work items of the given instances (longest first).
-void build_items(pv_batch *b, const std::vector<int> &instances) {
+bool wants_sym(const pv_batch *b) {
+    return b->sym_mode == 1 || (b->sym_mode < 0 && !b->adaptive);
+}
+
+// Symmetric work items: every tile pair (I <= J) of every listed instance, each instance's
+// partial sums placed after the previous one's, longest items first.
+void build_pair_items(pv_batch *b, const std::vector<int> &instances) {
+    struct Keyed {
+        double cost;
+        pv::PairItem item;
+    };
+    constexpr int TT = pv::kSymTile;
+    std::vector<Keyed> v;
+    size_t poff = 0;
+    for (int k : instances) {
+        pv::Instance &in = b->inst_h[k];
+        const int nt = (in.n + TT - 1) / TT;
+        in.nt = nt;
+        in.poff = poff;
+        poff += (size_t)nt * 2 * (size_t)in.n;
+        const double w = pair_weight(in.gp.geometry, b->precision);
+        for (int J = 0; J < nt; ++J) {
+            const double rj = std::min(TT, in.n - J * TT);
+            for (int I = 0; I <= J; ++I) {
+                const double ri = std::min(TT, in.n - I * TT);
+                v.push_back({w * ri * rj * (I == J ? 1.0 : 0.5), {k, I, J, 0}});
+            }
+        }
+    }
+    std::stable_sort(v.begin(), v.end(),
+                     [](const Keyed &a, const Keyed &c) { return a.cost > c.cost; });
+    // Chunks of at most kChunkItems consecutive items (every item has a fixed fetch and
+    // staging latency besides its pairs) and of at most total / kChunksWanted cost (an item
+    // costlier than that is a chunk by itself).
+    constexpr int kChunkItems = 4;
+    constexpr double kChunksWanted = 16384.0;
+    double total_cost = 0.0;
+    for (const Keyed &k : v)
+        total_cost += k.cost;
+    const double quota = total_cost / kChunksWanted;
+    std::vector<int> chunks{0};
+    double acc = 0.0;
+    for (size_t i = 0; i < v.size(); ++i) {
+        const size_t held = i - (size_t)chunks.back();
+        if (held > 0 && (held >= (size_t)kChunkItems || acc + v[i].cost > quota)) {
+            chunks.push_back((int)i);
+            acc = 0.0;
+        }
+        acc += v[i].cost;
+    }
+    if (!v.empty())
+        chunks.push_back((int)v.size());
+    if (v.size() > b->pitems_cap || poff > b->part_cap) {
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        if (v.size() > b->pitems_cap) {
+            dfree(b->pitems_d);
+            dfree(b->chunks_d);
+            b->pitems_cap = 0;
+            b->pitems_d = dalloc<pv::PairItem>(v.size(), "hipMalloc(work items)");
+            b->chunks_d = dalloc<int>(v.size() + 1, "hipMalloc(work chunks)");
+            b->pitems_cap = v.size();
+        }
+        if (poff > b->part_cap) {
+            dfree(b->part);
+            b->part_cap = 0;
+            b->part = dalloc<double>(poff, "hipMalloc(partial sums)");
+            b->part_cap = poff;
+        }
+    }
+    std::vector<pv::PairItem> items(v.size());
+    for (size_t i = 0; i < v.size(); ++i)
+        items[i] = v[i].item;
+    // The instance table is host-authoritative: the device never writes it.
+    hip_check(hipMemcpyAsync(b->inst_d, b->inst_h.data(), b->max_inst * sizeof(pv::Instance),
+                             hipMemcpyHostToDevice, b->stream),
+              "upload instances");
+    if (!items.empty()) {
+        hip_check(hipMemcpyAsync(b->pitems_d, items.data(), items.size() * sizeof(pv::PairItem),
+                                 hipMemcpyHostToDevice, b->stream),
+                  "upload work items");
+        hip_check(hipMemcpyAsync(b->chunks_d, chunks.data(), chunks.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, b->stream),
+                  "upload work chunks");
+    }
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    b->nitems = (int)items.size();
+    b->nchunks = items.empty() ? 0 : (int)chunks.size() - 1;
+}
+
+void build_items(pv_batch *b, const std::vector<int> &instances);
+
+// Cost-sorted work items of the given instances (longest first), ordered or symmetric.
+void build_ordered_items(pv_batch *b, const std::vector<int> &instances) {
     struct Keyed {
         double cost;
         pv::WorkItem item;
@@ -210,6 +326,15 @@ void build_items(pv_batch *b, const std::vector<int> &instances) {
         hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
     }
     b->nitems = (int)items.size();
+}
+
+void build_items(pv_batch *b, const std::vector<int> &instances) {
+    b->item_instances = instances;
+    b->sym = wants_sym(b);
+    if (b->sym)
+        build_pair_items(b, instances);
+    else
+        build_ordered_items(b, instances);
 }
 
 void read_status(pv_batch *b) {
@@ -251,6 +376,7 @@ void clear_instances(pv_batch *b) {
     b->nfixed = 0;
     b->nitems = 0;
     b->items_dirty = true;
+    b->item_instances.clear();
     b->steps_done = 0;
     hip_check(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), b->stream),
               "clear step count");
@@ -408,6 +534,28 @@ pv_status pv_batch_set_precision(pv_batch *b, int precision) {
     });
 }
 
+pv_status pv_batch_set_symmetric(pv_batch *b, int mode) {
+    return guard([&] {
+        checked(b);
+        if (mode < -1 || mode > 1)
+            raise(PV_ERR_INVALID_ARGUMENT, "symmetric mode must be -1, 0 or 1");
+        if (mode == b->sym_mode)
+            return;
+        DeviceGuard dg(b->device);
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        b->sym_mode = mode;
+        for (auto &kv : b->graphs)
+            (void)hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        if (b->adaptive)
+            b->items_dirty = true;
+        else if (!b->item_instances.empty())
+            build_items(b, std::vector<int>(b->item_instances));
+    });
+}
+
+int pv_batch_symmetric(const pv_batch *b) { return b && b->sym ? 1 : 0; }
+
 pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const size_t *counts,
                         size_t ninst, const double *x, const double *y, const double *gamma) {
     return guard([&] {
@@ -451,10 +599,11 @@ pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const si
         hip_check(hipMemcpyAsync(b->steps_d, b->steps_h, ninst * sizeof(pv::InstanceStep),
                                  hipMemcpyHostToDevice, b->stream),
                   "upload instance states");
-        build_items(b, all);
         b->total = total;
         b->nfixed = ninst;
         b->adaptive = false;
+        b->span = (int)total;
+        build_items(b, all);
         for (auto &kv : b->graphs)
             (void)hipGraphExecDestroy(kv.second);
         b->graphs.clear();
@@ -680,6 +829,7 @@ pv_status pv_batch_attempt(pv_batch *b) {
             if (b->used[k])
                 high = std::max(high, (size_t)b->inst_h[k].offset + (size_t)b->inst_h[k].n);
         const int n = (int)high;
+        b->span = n;
         pv::StageOp op{};
         for (int s = 0; s < 7; ++s) {
             op.kx[s] = b->kx[s], op.ky[s] = b->ky[s];

@@ -11,6 +11,7 @@
 // The stage arithmetic of RK4 / DOPRI5 is fused into the epilogue of the evaluation that
 // produces the stage (StageOp), so an RK4 step is four launches and a DOPRI5 attempt seven.
 #include "pv_epilogue.hpp"
+#include "pv_sym.hpp"
 
 namespace pv {
 
@@ -197,68 +198,6 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
 }
 
 // ------------------------------------------------------------------ symmetric evaluation
-
-// Sum of (r.u, r.v) over the 64 lanes of a wave; every lane gets the same bits (an xor
-// butterfly adds the same two values on both partners at every level).
-template <class A> __device__ __forceinline__ void wave_sum(A &r) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        r.u += __shfl_xor(r.u, off, 64);
-        r.v += __shfl_xor(r.v, off, 64);
-    }
-}
-
-// DPP wave_ror:1 (control 0x13C): every lane takes the value of its neighbour on one side, the
-// whole wave as one ring.  A double moves as two 32-bit halves.
-constexpr int kDppWaveRor1 = 0x13C;
-__device__ __forceinline__ double wave_ror1(double v) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, kDppWaveRor1, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), kDppWaveRor1, 0xF, 0xF, true);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
-                                            (unsigned long long)(unsigned)lo));
-}
-__device__ __forceinline__ float wave_ror1(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppWaveRor1, 0xF, 0xF, true));
-}
-
-// The ring's step in lane units (1 or 63, i.e. -1 mod 64): after k rotations lane l holds what
-// lane (l + k dir) mod 64 started with.  Read from the rotation itself, so the ring does not
-// depend on which way the hardware turns; the same for every lane.
-__device__ __forceinline__ int ring_dir(int lane) {
-    const int from = __builtin_amdgcn_mov_dpp(lane, kDppWaveRor1, 0xF, 0xF, true);
-    return __builtin_amdgcn_readfirstlane((from - lane) & 63);
-}
-
-// One source against a lane's targets: the target side into acc[t], the reaction on the source
-// into r.  The denominators of all TPT pairs share one reciprocal through the product tree.
-template <class P, int TPT>
-__device__ __forceinline__ void sym_step(const GeoParams &gp, const typename P::SymTgt *tg,
-                                         const typename P::SymScalar *gi,
-                                         const typename P::Src &s, typename P::Acc *acc,
-                                         typename P::SymR &r) {
-    if constexpr (P::kSymDenoms > 0) {
-        constexpr int ND = P::kSymDenoms;
-        typename P::SymTmp m[TPT];
-        double d[TPT * ND], inv[TPT * ND];
-#pragma unroll
-        for (int t = 0; t < TPT; ++t)
-            P::sym_prep(gp, tg[t], s, m[t], &d[t * ND]);
-        const bool ok = TreeInverse<TPT * ND>::run(d, inv);
-        if (tree_fallback(ok)) {
-#pragma unroll
-            for (int e = 0; e < TPT * ND; ++e)
-                inv[e] = rcp64(d[e]);
-        }
-#pragma unroll
-        for (int t = 0; t < TPT; ++t)
-            P::sym_apply(s, gi[t], m[t], &inv[t * ND], acc[t], r);
-    } else {
-#pragma unroll
-        for (int t = 0; t < TPT; ++t)
-            P::sym_pair(gp, tg[t], gi[t], s, acc[t], r);
-    }
-}
 
 // Block -> tile pair (I <= J) in row-major triangular order: b = J (J + 1) / 2 + I.
 __device__ __forceinline__ void tile_pair(int b, int &I, int &J) {

@@ -80,6 +80,15 @@ class Ensemble:
     def set_precision(self, precision: str):
         _check(self.lib.pv_batch_set_precision(self._h, PRECISIONS[precision]))
 
+    def set_symmetric(self, mode: int):
+        """1: each unordered pair once over tile pairs; 0: ordered 64-target items; -1
+        (default): symmetric for fixed-step ensembles, ordered for admitted runs."""
+        _check(self.lib.pv_batch_set_symmetric(self._h, int(mode)))
+
+    @property
+    def symmetric(self) -> bool:
+        return bool(self.lib.pv_batch_symmetric(self._h))
+
     # -- fixed-step ensembles
     def load(self, configs: Sequence[InstanceConfig], xs, ys, gs):
         counts = [len(x) for x in xs]

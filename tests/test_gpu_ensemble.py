@@ -46,10 +46,12 @@ def oracle_rk4(oracle, c, x, y, g, dt, steps):
     return x, y
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
-def test_ensemble_evaluate_matches_oracle(oracle, precision, tol):
+def test_ensemble_evaluate_matches_oracle(oracle, precision, tol, mode):
     cfgs, pops = mixed_population(30, 1)
     e = Ensemble(sum(p.n for p in pops), 64, precision)
+    e.set_symmetric(mode)
     e.load(cfgs, [p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
     us, vs = e.evaluate()
     for c, p, u, v in zip(cfgs, pops, us, vs):
@@ -120,17 +122,23 @@ def drive(ens, max_attempts):
     raise AssertionError(f"runs still running after {max_attempts} attempts: {ens.poll()}")
 
 
-def run_alone_and_mixed(precision):
+def run_alone_and_mixed(precision, mode=-1):
     rng = np.random.default_rng(7)
-    target = wl.periodic(64, 11, box_length=1.0)
+    target = wl.periodic(300 if mode == 1 else 64, 11, box_length=1.0)
+    if mode == 1:
+        # Three tiles of the symmetric decomposition; weaker circulations keep 300 vortices'
+        # closest pairs as slow as the 64-vortex run's.
+        target.circulation *= 0.01
     cfg = InstanceConfig("periodic", box_length=1.0)
     kw = dict(t_end=0.002, dt0=1e-4, atol=1e-10, rtol=1e-8)
     alone = Ensemble(4096, 8, precision)
+    alone.set_symmetric(mode)
     s0 = alone.admit(cfg, target.x, target.y, target.circulation, **kw)
-    drive(alone, 200)
+    drive(alone, 2000)
     st_alone = [s for s in alone.poll() if s.slot == s0][0]
     xa, ya = alone.fetch(s0, target.n)
     mix = Ensemble(20000, 16, precision)
+    mix.set_symmetric(mode)
     others = []
     for k in range(6):
         p = wl.disk(int(rng.integers(50, 900)), 50 + k, wall_fraction=0.0, gaussian=True)
@@ -145,9 +153,10 @@ def run_alone_and_mixed(precision):
     return st_alone, st_mix, (xa, ya), (xm, ym), mix.poll()
 
 
+@pytest.mark.parametrize("mode", [-1, 1])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_adaptive_run_alone_equals_run_in_the_mix(precision):
-    st_alone, st_mix, (xa, ya), (xm, ym), everyone = run_alone_and_mixed(precision)
+def test_adaptive_run_alone_equals_run_in_the_mix(precision, mode):
+    st_alone, st_mix, (xa, ya), (xm, ym), everyone = run_alone_and_mixed(precision, mode)
     assert st_alone.state == st_mix.state == "finished"
     assert st_alone.accepted == st_mix.accepted and st_alone.rejected == st_mix.rejected
     assert np.array_equal(xa, xm) and np.array_equal(ya, ym)
@@ -235,7 +244,8 @@ def test_serving_loop_completes_and_matches_alone():
         assert np.array_equal(r.x, by_id[req.rid].x) and np.array_equal(r.y, by_id[req.rid].y)
 
 
-def test_fixed_step_failures_are_per_instance(oracle):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fixed_step_failures_are_per_instance(oracle, mode):
     """A fixed-step instance that fails stops at the input state of its failing step and the
     others go on unchanged: a coincident pair fails in the first step, a dipole driven into
     the disk wall in the seventh.  Nine steps run as four two-step graph replays plus one
@@ -251,8 +261,11 @@ def test_fixed_step_failures_are_per_instance(oracle):
     wall = (InstanceConfig("disk"), wx, wy, wg)
     members = healthy[:3] + [coincident] + healthy[3:] + [wall]
     e = Ensemble(sum(len(m[1]) for m in members), 16)
+    e.set_symmetric(mode)
     e.load(*[list(t) for t in zip(*members)])
+    assert e.symmetric == bool(mode)
     ref = Ensemble(sum(len(m[1]) for m in healthy), 16)
+    ref.set_symmetric(mode)
     ref.load(*[list(t) for t in zip(*healthy)])
     dt = 2e-3
     e.rk4(dt, 9, use_graph=True)
@@ -281,16 +294,58 @@ def test_fixed_step_failures_are_per_instance(oracle):
         oracle.integrator(4).rk4_step(spec_disk, ox.copy(), oy.copy(), wg, dt)
 
 
-def test_ensemble_disk_near_centre_source(oracle):
-    """The ensemble kernel's per-wave tiles take the reference's image form when a source sits
-    within 2^-100 R of the centre."""
-    pops = [wl.disk(150, 61, wall_fraction=0.0), wl.disk(90, 62, wall_fraction=0.0)]
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ensemble_disk_near_centre_source(oracle, mode):
+    """The ensemble kernels' per-wave tiles take the reference's image form when a source sits
+    within 2^-100 R of the centre (symmetric mode: in tile pairs where that vortex is a target
+    and where it is a source)."""
+    pops = [wl.disk(150, 61, wall_fraction=0.0), wl.disk(90, 62, wall_fraction=0.0),
+            wl.disk(400, 63, wall_fraction=0.0)]
     pops[0].x[17], pops[0].y[17] = 1e-155, 0.0
     pops[1].x[80], pops[1].y[80] = 0.0, 0.0
-    cfgs = [InstanceConfig("disk"), InstanceConfig("disk")]
+    pops[2].x[200], pops[2].y[200] = -2e-160, 1e-161
+    cfgs = [InstanceConfig("disk")] * 3
     e = Ensemble(sum(p.n for p in pops), 4)
+    e.set_symmetric(mode)
     e.load(cfgs, [p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
     us, vs = e.evaluate()
     for c, p, u, v in zip(cfgs, pops, us, vs):
         ur, vr = oracle.evaluate(ospec(c), p.x, p.y, p.circulation)
         assert normwise(u, v, ur, vr) <= fp64_gate("disk", p.x, p.y, p.circulation, ur, vr)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_symmetric_and_ordered_ensembles_agree(precision):
+    """The two work decompositions of one ensemble (instances up to 5 tiles of 128): FP64
+    velocities within the FP64 bar of each other, mixed within the mixed bar of FP64;
+    symmetric repeats are bit-identical, and switching the mode keeps the loaded state."""
+    cfgs, pops = mixed_population(18, 21, n_min=2, n_max=620)
+    args = ([p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
+    total = sum(p.n for p in pops)
+    e = Ensemble(total, 32, precision)
+    e.load(cfgs, *args)
+    assert e.symmetric                      # fixed-step default
+    us, vs = e.evaluate()
+    us2, vs2 = e.evaluate()
+    assert all(np.array_equal(a, b) for a, b in zip(us, us2))
+    assert all(np.array_equal(a, b) for a, b in zip(vs, vs2))
+    e.set_symmetric(0)
+    assert not e.symmetric
+    uo, vo = e.evaluate()
+    ref = Ensemble(total, 32, "fp64")
+    ref.set_symmetric(0)
+    ref.load(cfgs, *args)
+    ur, vr = ref.evaluate()
+    bar = 1e-12 if precision == "fp64" else 1e-6
+    for k, (c, p) in enumerate(zip(cfgs, pops)):
+        assert normwise(us[k], vs[k], ur[k], vr[k]) <= bar, (k, c, p.n)
+        assert normwise(uo[k], vo[k], ur[k], vr[k]) <= bar, (k, c, p.n)
+    e.set_symmetric(1)
+    e.rk4(1e-4, 2, use_graph=True)
+    ref.set_symmetric(1)
+    ref.rk4(1e-4, 2, use_graph=False)
+    if precision == "fp64":
+        xa, ya = e.read()
+        xb, yb = ref.read()
+        assert all(np.array_equal(a, b) for a, b in zip(xa, xb))
+        assert all(np.array_equal(a, b) for a, b in zip(ya, yb))

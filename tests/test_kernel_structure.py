@@ -44,7 +44,7 @@ def batch_asm(tmp_path_factory):
 
 
 def _kernels(asm):
-    names = re.findall(r"\n(_ZN\w*batch_kernel\w*):", asm)
+    names = re.findall(r"\n(_ZN\w*batch_(?:sym_)?kernel\w*):", asm)
     for name in names:
         start = asm.index("\n" + name + ":")
         yield name, asm[start:asm.index("s_endpgm", start)].split("\n")
@@ -52,33 +52,68 @@ def _kernels(asm):
 
 def _loop_of_each_line(lines):
     """(header, depth) of the innermost loop that holds each line's basic block, from the
-    compiler's block annotations; None outside every loop."""
+    compiler's block annotations, and each loop header's parent; None outside every loop."""
     label = r"^(?:\.LBB\d+_\d+:|; %bb\.\d+:)"
-    cur, out = None, []
+    cur, out, parent, pending = None, [], {}, []
     for line in lines:
         m = re.match(r"^\.(LBB\d+_\d+):\s*;\s*=>\s*This (?:Inner )?Loop Header: Depth=(\d+)", line)
         if m:
             cur = (m.group(1)[1:], int(m.group(2)))
-        else:
-            m = re.match(label + r"\s*;\s*in Loop: Header=(BB\d+_\d+) Depth=(\d+)", line)
-            if m:
-                cur = (m.group(1), int(m.group(2)))
-            elif re.match(label, line):
-                cur = None
+            parent[cur[0]] = None
+            pending = []
+            out.append(cur)
+            continue
+        m = re.match(r"^\.(LBB\d+_\d+):\s*;\s*Parent Loop (BB\d+_\d+) Depth=(\d+)", line)
+        if m:
+            # A nested loop header: its ancestors follow, then the header line itself.
+            pending = [(m.group(2), int(m.group(3)))]
+            cur = (m.group(1)[1:], None)
+            out.append(cur)
+            continue
+        m = re.match(r"^\s*;\s*Parent Loop (BB\d+_\d+) Depth=(\d+)", line)
+        if m and pending:
+            pending.append((m.group(1), int(m.group(2))))
+            out.append(cur)
+            continue
+        m = re.match(r"^\s*;\s*=>\s*This (?:Inner )?Loop Header: Depth=(\d+)", line)
+        if m and pending:
+            depth = int(m.group(1))
+            parent[cur[0]] = next(h for h, d in pending if d == depth - 1)
+            cur = (cur[0], depth)
+            pending = []
+            out[-1] = cur
+            out.append(cur)
+            continue
+        m = re.match(label + r"\s*;\s*in Loop: Header=(BB\d+_\d+) Depth=(\d+)", line)
+        if m:
+            cur = (m.group(1), int(m.group(2)))
+        elif re.match(label, line):
+            cur = None
         out.append(cur)
-    return out
+    return out, parent
 
 
 def test_every_persistent_kernel_was_found(batch_asm):
     names = [n for n, _ in _kernels(batch_asm)]
-    assert len(names) == 4, names      # {fp64, mixed} x {fixed, adaptive}
+    assert len(names) == 8, names      # {ordered, symmetric} x {fp64, mixed} x {fixed, adaptive}
 
 
 def test_item_fetch_and_item_load_share_the_outermost_loop(batch_asm):
+    """The fetch sits in the outermost loop; the item load sits in that same loop (ordered
+    kernels: one item per fetch) or in a loop nested directly in it (symmetric kernels: one
+    chunk of consecutive items per fetch, its bounds wave-uniform)."""
     for name, lines in _kernels(batch_asm):
-        loops = _loop_of_each_line(lines)
+        loops, parent = _loop_of_each_line(lines)
         fetch = [i for i, l in enumerate(lines) if "global_atomic_add" in l]
         assert len(fetch) == 1, (name, fetch)
-        load = next(i for i, l in enumerate(lines) if i > fetch[0] and "global_load" in l)
-        assert loops[fetch[0]] is not None and loops[fetch[0]][1] == 1, (name, loops[fetch[0]])
-        assert loops[load] == loops[fetch[0]], (name, loops[fetch[0]], loops[load])
+        loads = [i for i, l in enumerate(lines) if i > fetch[0] and "global_load" in l]
+        outer = loops[fetch[0]]
+        assert outer is not None and outer[1] == 1, (name, outer)
+        if "batch_sym_kernel" in name:
+            # The chunk's bounds are read where the chunk was fetched; its items in the loop
+            # directly inside that one.
+            assert loops[loads[0]] == outer, (name, outer, loops[loads[0]])
+            item = next(i for i in loads if loops[i] is not None and loops[i][1] == 2)
+            assert parent[loops[item][0]] == outer[0], (name, loops[item], outer)
+        else:
+            assert loops[loads[0]] == outer, (name, outer, loops[loads[0]])
