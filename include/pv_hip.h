@@ -245,6 +245,9 @@ size_t pv_batch_bytes_per_vortex(void);
  * current work items use. */
 pv_status pv_batch_set_symmetric(pv_batch *b, int mode);
 int pv_batch_symmetric(const pv_batch *b);
+/* Partial-sum bytes the symmetric decomposition allocates for one instance of n vortices,
+ * on top of n x pv_batch_bytes_per_vortex(). */
+size_t pv_batch_symmetric_bytes(size_t n);
 
 /* Fixed-step ensembles: replace the contents with `ninst` instances laid out one after the
  * other in x, y, gamma (counts[k] vortices each), advance them with RK4 (optionally as

@@ -165,6 +165,7 @@ _SIGNATURES = {
     "pv_batch_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pv_batch_set_symmetric": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pv_batch_symmetric": (ctypes.c_int, [_vp]),
+    "pv_batch_symmetric_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),
     "pv_batch_bytes_per_vortex": (ctypes.c_size_t, []),
     "pv_batch_load": (ctypes.c_int, [_vp, ctypes.POINTER(PvInstanceConfig), _szp,
                                      ctypes.c_size_t, _dp, _dp, _dp]),

@@ -434,6 +434,11 @@ extern "C" {
 
 size_t pv_batch_bytes_per_vortex(void) { return kDoublesPerVortex * sizeof(double) + 8; }
 
+size_t pv_batch_symmetric_bytes(size_t n) {
+    const size_t tiles = (n + pv::kSymTile - 1) / pv::kSymTile;
+    return tiles * 2 * n * sizeof(double);
+}
+
 pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max_instances,
                           pv_batch **out) {
     return guard([&] {

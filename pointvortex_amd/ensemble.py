@@ -54,6 +54,11 @@ def bytes_per_vortex() -> int:
     return int(load_library().pv_batch_bytes_per_vortex())
 
 
+def symmetric_bytes(n: int) -> int:
+    """Partial-sum bytes of one n-vortex instance in the symmetric decomposition."""
+    return int(load_library().pv_batch_symmetric_bytes(int(n)))
+
+
 class Ensemble:
     def __init__(self, capacity: int, max_instances: int, precision: str = "fp64",
                  device: int = 0):

@@ -1,11 +1,14 @@
 """A request loop with continuous batching over adaptive ensembles.
 
 Requests (geometry, N, circulations, horizon, tolerance) arrive over time.  Between DOPRI5
-attempts the loop admits every arrived request whose vortices fit (the HBM budget is the
-ensemble capacity, 208 bytes per vortex), so new runs join runs already in flight; finished or
-failed runs leave and free their slots.  Each run keeps its state, FSAL derivative and
-step-size controller on the device; one attempt advances every running run by one DOPRI5
-attempt in seven persistent launches.  Latency is measured from arrival to completion.
+attempts the loop admits, in arrival order, every arrived request whose HBM footprint fits the
+budget -- 208 bytes per vortex of ensemble state, plus the partial sums of the symmetric
+decomposition (16 bytes per vortex per 128 vortices of the run) -- so new runs join runs
+already in flight; finished or failed runs leave and free their bytes.  Each run keeps its
+state, FSAL derivative and step-size controller on the device; one attempt advances every
+running run by one DOPRI5 attempt (seven stage evaluations).  Runs are split into pairs of
+128-vortex tiles, each unordered pair computed once (`Ensemble.set_symmetric`), so a lone
+large run still spreads over the GPU.  Latency is measured from arrival to completion.
 
 Precision policy: a request runs in mixed precision when its rtol is at least
 `mixed_rtol_threshold` (1e-6 by default), otherwise in FP64; each precision has its own
@@ -21,7 +24,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import workloads as wl
-from .ensemble import Ensemble, InstanceConfig, bytes_per_vortex
+from .ensemble import Ensemble, InstanceConfig, bytes_per_vortex, symmetric_bytes
 
 
 @dataclass
@@ -86,22 +89,32 @@ class ServerStats:
 
 class Server:
     def __init__(self, hbm_budget_bytes: int, max_runs: int = 4096, device: int = 0,
-                 mixed_rtol_threshold: float = 1e-6, keep_results: bool = False):
-        per = bytes_per_vortex()
-        self.capacity = max(1, int(hbm_budget_bytes // per))
+                 mixed_rtol_threshold: float = 1e-6, keep_results: bool = False,
+                 symmetric: bool = True):
+        self.per_vortex = bytes_per_vortex()
+        self.budget = int(hbm_budget_bytes)
+        self.capacity = max(1, self.budget // self.per_vortex)
+        self.used = 0                      # footprint of the admitted runs, both precisions
+        self.peak_used = 0
         self.max_runs = max_runs
         self.device = device
         self.threshold = mixed_rtol_threshold
         self.keep = keep_results
+        self.symmetric = symmetric
         self.engines: Dict[str, Ensemble] = {}
+
+    def footprint(self, n: int) -> int:
+        """HBM bytes a run of n vortices holds while admitted."""
+        return n * self.per_vortex + (symmetric_bytes(n) if self.symmetric else 0)
 
     def precision_for(self, req: Request) -> str:
         return "mixed" if req.rtol >= self.threshold else "fp64"
 
     def engine(self, precision: str) -> Ensemble:
         if precision not in self.engines:
-            self.engines[precision] = Ensemble(self.capacity, self.max_runs, precision,
-                                               self.device)
+            e = Ensemble(self.capacity, self.max_runs, precision, self.device)
+            e.set_symmetric(1 if self.symmetric else 0)
+            self.engines[precision] = e
         return self.engines[precision]
 
     def run(self, requests: List[Request], clock=time.perf_counter,
@@ -121,13 +134,18 @@ class Server:
             while pending and pending[0].arrival <= now:
                 req = pending[0]
                 prec = self.precision_for(req)
-                pop, cfg = req.population()
-                slot = self.engine(prec).admit(cfg, pop.x, pop.y, pop.circulation, req.t_end,
-                                               req.dt0, req.atol, req.rtol)
+                need = self.footprint(req.n)
+                slot = None
+                if self.used + need <= self.budget:
+                    pop, cfg = req.population()
+                    slot = self.engine(prec).admit(cfg, pop.x, pop.y, pop.circulation,
+                                                   req.t_end, req.dt0, req.atol, req.rtol)
                 if slot is None:
-                    if not any(k[0] == prec for k in active):
+                    if not active:
                         raise RuntimeError(f"request {req.rid} (N={req.n}) exceeds the budget")
                     break
+                self.used += need
+                self.peak_used = max(self.peak_used, self.used)
                 active[(prec, slot)] = (req, clock() - t0)
                 pending.popleft()
             if not active:
@@ -150,6 +168,7 @@ class Server:
                     if self.keep and s.state == "finished":
                         x, y = eng.fetch(s.slot, s.n)
                     eng.release(s.slot)
+                    self.used -= self.footprint(req.n)
                     stats.results.append(Result(req.rid, s.state, req.arrival, admitted, now,
                                                 s.accepted, s.rejected, s.failure, prec, x, y))
                     if on_result is not None:
