@@ -256,6 +256,93 @@ def c2_slice(pv, accepted=40, budget_s=20.0):
     return out
 
 
+def c1_whole(pv, steps=1000, dt=1e-3):
+    """The infinite-plane configuration whole on the device: N=1024, +-1 (seed 0), RK4
+    dt=1e-3 for 1000 steps, FP64 and mixed; 4 evaluations per step, N(N-1) pairs each."""
+    from pointvortex_amd import workloads as wl
+
+    pop = wl.config1()
+    n = pop.n
+    out = {"n": n, "steps": steps, "dt": dt}
+    for prec in ("fp64", "mixed"):
+        k = pv.HipKernel(pv.SimParams(boundaryCondition="infinite", precision=prec))
+        s = pv.VortexSystem(pop.x, pop.y, pop.circulation)
+        k.uploadDeviceState(s)
+        k.deviceRk4Advance(dt, 10)
+        k.uploadDeviceState(s)
+        k.synchronize()
+        t0 = time.perf_counter()
+        k.deviceRk4Advance(dt, steps)
+        k.synchronize()
+        wall = time.perf_counter() - t0
+        out[prec] = {"wall_s": wall, "pairs_per_s": 4 * steps * n * (n - 1) / wall,
+                     "launch_shape": list(k.resolved_launch(n, n))}
+        k.close()
+    log(f"[bench] c1 whole: fp64 {out['fp64']['wall_s']:.3f} s, "
+        f"mixed {out['mixed']['wall_s']:.3f} s for {steps} steps")
+    return out
+
+
+def cpu_samples(threads):
+    """The CPU beside the GPU for the other configurations, each a bounded sample timed with
+    the reference's own CPU code (oracle/_ref, else the C restatement): the infinite-plane
+    run whole (RK4, 1000 steps at N=1024); 1024 of the 16384 targets of one periodic
+    evaluation; one evaluation of 64 ensemble instances stratified by geometry (32 infinite,
+    16 periodic, 16 disk) and by N."""
+    from oracle import pyoracle
+    from pointvortex_amd import workloads as wl
+
+    ref = pyoracle.Reference.available()
+    impl = pyoracle.Reference() if ref else pyoracle.Oracle()
+    kind = "reference" if ref else "port"
+    out = {}
+    pop = wl.config1()
+    k = pyoracle.spec("infinite")
+    t0 = time.perf_counter()
+    if ref:
+        impl.rk4(k, pop.x, pop.y, pop.circulation, 1e-3, 1000)
+    else:
+        it = impl.integrator(pop.n)
+        x, y = pop.x.copy(), pop.y.copy()
+        for _ in range(1000):
+            it.rk4_step(k, x, y, pop.circulation, 1e-3)
+    wall = time.perf_counter() - t0
+    n = pop.n
+    out["c1_whole"] = {"value": 4000 * n * (n - 1) / wall, "unit": "pair-interactions/s",
+                       "wall_s": wall, "sample": "the whole run: RK4, 1000 steps at N=1024"}
+    pop = wl.config2()
+    k = pyoracle.spec("periodic", box_length=2 * np.pi, image_layers=8)
+    t0 = time.perf_counter()
+    impl.evaluate_range(k, pop.x, pop.y, pop.circulation, 0, 1024)
+    wall = time.perf_counter() - t0
+    out["c2_targets"] = {"value": 1024 * (pop.n - 1) / wall, "unit": "pair-interactions/s",
+                         "wall_s": wall,
+                         "sample": f"targets 0..1023 of one N={pop.n} periodic evaluation "
+                                   "(8 image layers)"}
+    pops = wl.ensemble(4096)
+    pick = []
+    for geo, m in (("infinite", 32), ("periodic", 16), ("disk", 16)):
+        idx = sorted((i for i, p in enumerate(pops) if p.geometry == geo),
+                     key=lambda i: pops[i].n)
+        pick += [idx[int(q)] for q in np.linspace(0, len(idx) - 1, m)]
+    pairs, t0 = 0, time.perf_counter()
+    for i in pick:
+        p = pops[i]
+        ks = pyoracle.spec(p.geometry, box_length=getattr(p, "box_length", 1.0),
+                           radius=getattr(p, "radius", 1.0))
+        impl.evaluate(ks, p.x, p.y, p.circulation)
+        pairs += p.n * (p.n - 1)
+    wall = time.perf_counter() - t0
+    out["c4_instances"] = {"value": pairs / wall, "unit": "pair-interactions/s",
+                           "wall_s": wall,
+                           "sample": "one evaluation of 64 of the 4096 instances (32 infinite, "
+                                     "16 periodic, 16 disk, spread over N)"}
+    for v in out.values():
+        v.update({"cores": threads, "kind": kind})
+    log(f"[bench] CPU samples: " + ", ".join(f"{k} {v['value']:.3e}" for k, v in out.items()))
+    return out
+
+
 def serving_bench(pv, count=128, seed=5):
     """Continuous-batching request loop, scaled down so that it runs in about a minute:
     N log-uniform in [64, 2048], horizon U[1e-4, 1e-3], rtol log-uniform in [1e-10, 1e-6],
@@ -324,7 +411,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the per-geometry table, the ensemble and the serving loop")
     ap.add_argument("--ensemble-steps", type=int, default=500)
-    ap.add_argument("--only", choices=["fp64", "geometries", "copy", "ensemble", "serving", "c2"],
+    ap.add_argument("--only", choices=["fp64", "geometries", "copy", "ensemble", "serving", "c2",
+                                       "c1"],
                     default=None,
                     help="profiling runs: only the FP64 RK4 run, only the per-geometry table, "
                          "or only the 256 MiB copy-kernel calibration")
@@ -354,6 +442,9 @@ def main():
         return
     if args.only == "c2":
         print(json.dumps(c2_slice(pv)), flush=True)
+        return
+    if args.only == "c1":
+        print(json.dumps(c1_whole(pv)), flush=True)
         return
     pop = wl.disk(n, 2)
     pairs_per_step = 4 * n * (n - 1)
@@ -441,6 +532,7 @@ def main():
     }
     if not args.no_extra:
         result["per_geometry_n65536"] = per_geometry(pv, n)
+        result["infinite_n1024_rk4_whole"] = c1_whole(pv)
         result["periodic_n16384_dopri5_slice"] = c2_slice(pv)
         result["ensemble_4096"] = ensemble_bench(pv, args.ensemble_steps)
         result["serving"] = serving_bench(pv)
@@ -448,6 +540,11 @@ def main():
         log("[bench] CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(pop)
         log(f"[bench] CPU: {result['cpu_baseline']['value']:.3e} pairs/s")
+        if not args.no_extra:
+            samples = cpu_samples(result["cpu_baseline"]["cores"])
+            result["infinite_n1024_rk4_whole"]["cpu"] = samples["c1_whole"]
+            result["periodic_n16384_dopri5_slice"]["cpu"] = samples["c2_targets"]
+            result["ensemble_4096"]["cpu"] = samples["c4_instances"]
     print(json.dumps(result), flush=True)
 
 

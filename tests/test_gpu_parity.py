@@ -441,3 +441,25 @@ def test_mixed_parity_at_n65536_on_target_subsets(oracle, geometry):
     for b, e in [(0, 256), (n - 256, n)]:
         ur, vr = oracle.evaluate_range(s, pop.x, pop.y, pop.circulation, b, e)
         assert normwise(vel.x[b:e], vel.y[b:e], ur[b:e], vr[b:e]) <= 1e-6, (b, e)
+
+
+def test_mixed_disk_run_against_the_oracle_at_first_and_last_step(oracle):
+    """The N=65536 disk RK4 run of the benchmark (dt=1e-4, 200 steps) in mixed precision: the
+    mixed kernel's velocities on the run's first and last states against the oracle's FP64
+    sums for 512 targets (the first 256 and the wall shell, the last 256), normwise 1e-6."""
+    n = 65536
+    pop = wl.disk(n, 2)
+    k = kernel_for("disk", "mixed")
+    s = VortexSystem(pop.x, pop.y, pop.circulation)
+    k.uploadDeviceState(s)
+    states = [s.copy()]
+    k.deviceRk4Advance(1e-4, 200)
+    last = s.copy()
+    k.downloadDeviceState(last)
+    states.append(last)
+    for st in states:
+        assert np.all(np.hypot(st.x, st.y) < 1.0)
+        vel = k.evaluate(VortexSystem(st.x, st.y, pop.circulation))
+        for b, e in [(0, 256), (n - 256, n)]:
+            ur, vr = oracle.evaluate_range(spec("disk"), st.x, st.y, pop.circulation, b, e)
+            assert normwise(vel.x[b:e], vel.y[b:e], ur[b:e], vr[b:e]) <= 1e-6, (b, e)
