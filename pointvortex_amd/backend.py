@@ -237,14 +237,18 @@ class SimParams:
 
 @dataclass
 class VortexSystem:
-    """src/vortex.h:15-38: structure-of-arrays FP64 state."""
+    """src/vortex.h:15-38: structure-of-arrays FP64 state.  Like the reference's struct of
+    std::vectors it owns its arrays: construction copies them, so downloadDeviceState into a
+    system never writes through to the arrays it was built from."""
 
     x: np.ndarray = field(default_factory=lambda: np.zeros(0))
     y: np.ndarray = field(default_factory=lambda: np.zeros(0))
     circulation: np.ndarray = field(default_factory=lambda: np.zeros(0))
 
     def __post_init__(self):
-        self.x, self.y, self.circulation = _f64(self.x), _f64(self.y), _f64(self.circulation)
+        self.x = np.array(self.x, dtype=np.float64, order="C", copy=True)
+        self.y = np.array(self.y, dtype=np.float64, order="C", copy=True)
+        self.circulation = np.array(self.circulation, dtype=np.float64, order="C", copy=True)
 
     def size(self):
         return len(self.x)

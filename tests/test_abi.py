@@ -29,3 +29,15 @@ def test_header_is_plain_c():
 def test_last_error_is_a_string():
     lib = backend.load_library()
     assert isinstance(lib.pv_last_error(), bytes)
+
+
+def test_vortex_system_owns_its_arrays():
+    """A system copies what it is built from (the reference's struct holds std::vectors), so
+    writing a downloaded state into it leaves the caller's arrays unchanged."""
+    import numpy as np
+
+    x, y, g = np.arange(4.0), np.ones(4), np.full(4, 0.5)
+    s = backend.VortexSystem(x, y, g)
+    s.x[:] = -1.0
+    assert np.array_equal(x, np.arange(4.0))
+    assert s.x.flags["C_CONTIGUOUS"] and s.x.dtype == np.float64
