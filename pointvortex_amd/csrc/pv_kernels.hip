@@ -624,28 +624,31 @@ __global__ __launch_bounds__(kReduceBlock) void energy_kernel(GeoParams gp, cons
 
 // Close opposite-sign pairs (src/dipole.cpp:90-118), with the reference's non-fused
 // arithmetic (contraction off) so that candidate sets and their d^2 match it bit for bit.
+// One block per pair of 256-vortex tiles (I <= J): targets i of tile I against sources j > i
+// of tile J, so the O(N^2 / 2) tests spread evenly over nt (nt + 1) / 2 blocks; the disk wall
+// test runs once per vortex, in the diagonal blocks.  Candidates are appended with an atomic
+// counter (their order is not significant: the host sorts them).
 __global__ __launch_bounds__(kReduceBlock) void dipole_kernel(
     GeoParams gp, int periodic, int disk, double thr2, const double *x, const double *y,
     const double *g, int n, unsigned long long *first, unsigned long long *second, double *d2,
     unsigned long long cap, DevStatus *st) {
 #pragma clang fp contract(off)
     __shared__ double sx[kReduceBlock], sy[kReduceBlock], sg[kReduceBlock];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int I, J;
+    tile_pair(blockIdx.x, I, J);
+    const int i = I * kReduceBlock + (int)threadIdx.x;
     const bool valid = i < n;
     const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, gi = valid ? g[i] : 0.0;
     const bool active = valid && gi != 0.0;
-    const int first_j = blockIdx.x * blockDim.x;   // sources below the block never pair j > i
-    for (int base = first_j; base < n; base += kReduceBlock) {
-        const int cnt = min(kReduceBlock, n - base);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            sx[threadIdx.x] = x[base + threadIdx.x];
-            sy[threadIdx.x] = y[base + threadIdx.x];
-            sg[threadIdx.x] = g[base + threadIdx.x];
-        }
-        __syncthreads();
-        if (!active)
-            continue;
+    const int base = J * kReduceBlock;
+    const int cnt = min(kReduceBlock, n - base);
+    if ((int)threadIdx.x < cnt) {
+        sx[threadIdx.x] = x[base + threadIdx.x];
+        sy[threadIdx.x] = y[base + threadIdx.x];
+        sg[threadIdx.x] = g[base + threadIdx.x];
+    }
+    __syncthreads();
+    if (active) {
         for (int k = 0; k < cnt; ++k) {
             const int j = base + k;
             if (j <= i)
@@ -669,7 +672,7 @@ __global__ __launch_bounds__(kReduceBlock) void dipole_kernel(
             }
         }
     }
-    if (active && disk) {
+    if (active && disk && I == J) {
         const double ri2 = xi * xi + yi * yi;
         if (ri2 > 0.0) {
             const double idist = (gp.R2 - ri2) / sqrt(ri2);
@@ -825,9 +828,10 @@ void launch_dipoles(const GeoParams &gp, bool periodic, bool disk, double thr2, 
                     const double *y, const double *g, int n, unsigned long long *first,
                     unsigned long long *second, double *d2, unsigned long long cap,
                     DevStatus *st, hipStream_t s) {
-    hipLaunchKernelGGL(dipole_kernel, dim3((n + kReduceBlock - 1) / kReduceBlock),
-                       dim3(kReduceBlock), 0, s, gp, (int)periodic, (int)disk, thr2, x, y, g, n,
-                       first, second, d2, cap, st);
+    const size_t nt = ((size_t)n + kReduceBlock - 1) / kReduceBlock;
+    hipLaunchKernelGGL(dipole_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(kReduceBlock), 0,
+                       s, gp, (int)periodic, (int)disk, thr2, x, y, g, n, first, second, d2, cap,
+                       st);
 }
 
 void launch_copy(const double4 *src, double4 *dst, size_t count, hipStream_t s) {
