@@ -479,7 +479,7 @@ def main():
     avg_ms = kms64 / launches64
     achieved = n * (n - 1) * FLOPS_PER_PAIR["disk"] / (avg_ms / 1e3) / 1e12
     tpt, splits, tile, sym = k64.resolved_launch(n, n)
-    kernels = (f"sym_kernel<DiskF64,{tpt},{tile}> + sym_reduce_kernel<DiskF64>" if sym else
+    kernels = (f"sym_kernel<DiskF64,{tpt},{tile}> + sym_reduce_kernel" if sym else
                f"pair_kernel<DiskF64,{tpt},{tile}> ({splits} source splits)")
     result = {
         "metric": METRIC,

@@ -195,3 +195,13 @@ def test_symmetric_path_in_rk4_and_fallbacks(oracle, geometry):
     km.set_launch(2, 0, 256, 1)
     assert km.resolved_launch(3000, 3000)[3] == 1
     assert km.resolved_launch(100, 3000)[3] == 0
+
+
+def test_symmetric_path_respects_the_partial_limit():
+    """A forced symmetric shape runs while its partial sums (nt x 2N doubles) stay under 8 GiB
+    and falls back to the split path above (resolution only; nothing is allocated)."""
+    _, k = setup("disk", "fp64", 64)
+    k.set_launch(4, 0, 128, 1)
+    assert k.resolved_launch(262144, 262144)[3] == 1          # 256 rows x 2N: 1.1 GB
+    assert k.resolved_launch(10 ** 6, 10 ** 6)[3] == 0        # 977 rows x 2N: 15.6 GB
+    assert k.resolved_launch(1000, 10 ** 6)[3] == 0           # target subsets never
