@@ -221,9 +221,11 @@ def ensemble_bench(pv, steps, count=4096):
 def c2_slice(pv, accepted=40, budget_s=20.0):
     """The periodic configuration (L=2 pi, N=16384, +-1, seed 1) as a slice: DOPRI5 at rtol
     1e-9, atol 1e-12 from dt=1e-3 for `accepted` accepted steps (or `budget_s`), with the
-    device dipole-candidate search (removal distance 1e-3 L) after every accepted step, as
-    the driver's event check runs it.  Pairs are metered as N(N-1) per evaluation: six per
-    attempt after the first (FSAL), seven for the first."""
+    driver's dipole events after every accepted step (src/main.cpp:327-337): removal below
+    1e-3 L and independent reinjection (randomSeed 1234567, src/params.h:64) on the device
+    state, and the segment invariants when an event happened.  Pairs are metered as N(N-1)
+    per evaluation: six per attempt after the first (FSAL), seven when the derivative was
+    invalidated by an event."""
     from pointvortex_amd import workloads as wl
 
     pop = wl.config2()
@@ -231,27 +233,36 @@ def c2_slice(pv, accepted=40, budget_s=20.0):
     k = pv.HipKernel(pv.SimParams(boundaryCondition="periodic", boxLengthX=2 * np.pi,
                                   boxLengthY=2 * np.pi))
     k.uploadDeviceState(pv.VortexSystem(pop.x, pop.y, pop.circulation))
-    dt, t, steps, attempts, cand = 1e-3, 0.0, 0, 0, 0
-    search_s = 0.0
+    k.setDipoleRandom(1234567)
+    dt, t, steps, attempts, events, fresh = 1e-3, 0.0, 0, 0, 0, 1
+    event_s = 0.0
+    dts = []
     t0 = time.perf_counter()
     while steps < accepted and time.perf_counter() - t0 < budget_s:
         r = k.deviceDopri5Step(dt, 1e-12, 1e-9, 1e-12, 0.1)
         attempts += 1 + r.rejectedSteps
         steps += 1
         t += r.acceptedTimeStep
+        dts.append(r.acceptedTimeStep)
         dt = r.suggestedTimeStep
         s0 = time.perf_counter()
-        cand += len(k.dipoleCandidates(1e-3 * 2 * np.pi)[0])
-        search_s += time.perf_counter() - s0
+        e = k.processDipoles(1e-3 * 2 * np.pi, "independent")
+        if e:
+            events += e
+            fresh += 1
+            k.deviceInvariants()
+        event_s += time.perf_counter() - s0
     wall = time.perf_counter() - t0
-    evals = 6 * attempts + 1
+    evals = 6 * attempts + fresh
     out = {"n": n, "accepted_steps": steps, "attempts": attempts, "t_reached": t,
            "wall_s": wall, "attempts_per_s": attempts / wall,
            "pairs_per_s": evals * n * (n - 1) / wall,
-           "dipole_search_s_per_step": search_s / max(1, steps),
-           "dipole_candidates_seen": cand}
-    log(f"[bench] c2 slice: {steps} accepted steps, {attempts} attempts in {wall:.2f} s "
-        f"({out['pairs_per_s']:.3e} pairs/s), t={t:.3e}")
+           "dipole_events": events,
+           "dipole_event_s_per_step": event_s / max(1, steps),
+           "accepted_dt_first_last": [dts[0], dts[-1]] if dts else None}
+    log(f"[bench] c2 slice: {steps} accepted steps, {attempts} attempts, {events} dipole "
+        f"events in {wall:.2f} s ({out['pairs_per_s']:.3e} pairs/s), t={t:.3e}, "
+        f"dt {dts[0]:.2e} -> {dts[-1]:.2e}")
     k.close()
     return out
 

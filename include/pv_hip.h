@@ -152,6 +152,25 @@ pv_status pv_state_invariants(pv_handle *h, pv_invariants *out);
 pv_status pv_dipole_candidates(pv_handle *h, double distance, uint64_t *first, uint64_t *second,
                                double *distance_squared, size_t cap, size_t *count);
 
+/* ReinjectionMode (src/params.h:10). */
+enum pv_reinjection { PV_REINJECT_NONE = 0, PV_REINJECT_INDEPENDENT = 1, PV_REINJECT_PAIRED = 2 };
+
+/* DipoleManager::process (src/dipole.cpp:42-125) on the resident state: the candidate search
+ * above, then on the host the closest-first disjoint selection, the compaction (survivors in
+ * index order) and the reinjection (src/dipole.cpp:127-176) with the handle's generator; the
+ * new state (its size can change) replaces the resident one and the FSAL derivative is
+ * invalidated.  *events receives the number of removed pairs (wall events included).  The
+ * driver would call it where it calls DipoleManager::process on host vectors. */
+pv_status pv_dipole_process(pv_handle *h, double distance, int reinjection, size_t *events);
+
+/* The reinjection generator: DipoleManager's constructors (src/dipole.cpp:29-38): seeded with
+ * seed ^ 0xd1b54a32d192ed03, or restored from `state` (the text std::mt19937_64 writes, as
+ * stored in checkpoints, src/checkpoint.cpp:66-68) when it is non-empty.
+ * pv_dipole_random_state writes that text (DipoleManager::state, src/dipole.cpp:178-182);
+ * *needed receives its length plus one. */
+pv_status pv_dipole_set_random(pv_handle *h, uint64_t seed, const char *state);
+pv_status pv_dipole_random_state(pv_handle *h, char *buffer, size_t length, size_t *needed);
+
 /* backendRuntimeDetails (src/backend.h:14): newline-separated "key value" lines. */
 pv_status pv_runtime_details(int device, char *buffer, size_t length);
 

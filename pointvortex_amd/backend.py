@@ -164,6 +164,11 @@ _SIGNATURES = {
     "pv_batch_destroy": (None, [_vp]),
     "pv_batch_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pv_batch_set_symmetric": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pv_dipole_process": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_size_t)]),
+    "pv_dipole_set_random": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_char_p]),
+    "pv_dipole_random_state": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t,
+                                              ctypes.POINTER(ctypes.c_size_t)]),
     "pv_batch_symmetric": (ctypes.c_int, [_vp]),
     "pv_batch_symmetric_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),
     "pv_batch_bytes_per_vortex": (ctypes.c_size_t, []),
@@ -391,6 +396,15 @@ class HipKernel:
         n = vortices.size()
         _check(_lib.pv_download(self._h, _ptr(vortices.x), _ptr(vortices.y), None, n))
 
+    def downloadDeviceSystem(self) -> VortexSystem:
+        """The whole resident system, circulations included (after processDipoles they are
+        no longer the uploaded ones)."""
+        n = self.stateSize()
+        s = VortexSystem(np.zeros(n), np.zeros(n), np.zeros(n))
+        if n:
+            _check(_lib.pv_download(self._h, _ptr(s.x), _ptr(s.y), _ptr(s.circulation), n))
+        return s
+
     def stateSize(self) -> int:
         n = ctypes.c_size_t(0)
         _check(_lib.pv_state_size(self._h, ctypes.byref(n)))
@@ -438,6 +452,29 @@ class HipKernel:
             return self.dipoleCandidates(distance, count.value)
         c = count.value
         return first[:c].copy(), second[:c].copy(), d2[:c].copy()
+
+    REINJECTION = {"none": 0, "independent": 1, "paired": 2}
+
+    def processDipoles(self, distance: float, reinjection: str = "none") -> int:
+        """DipoleManager::process (src/dipole.cpp:42-125) on the device state: the search on
+        the device, then the reference's selection, compaction and seeded reinjection; the
+        resident state may change size.  Returns the number of removed pairs."""
+        events = ctypes.c_size_t(0)
+        _check(_lib.pv_dipole_process(self._h, float(distance), self.REINJECTION[reinjection],
+                                      ctypes.byref(events)))
+        return events.value
+
+    def setDipoleRandom(self, seed: int, state: str = ""):
+        """The reinjection generator: seeded like DipoleManager (seed ^ 0xd1b54a32d192ed03),
+        or restored from a checkpoint's generator text."""
+        _check(_lib.pv_dipole_set_random(self._h, int(seed) & (2 ** 64 - 1), state.encode()))
+
+    def dipoleRandomState(self) -> str:
+        need = ctypes.c_size_t(0)
+        _check(_lib.pv_dipole_random_state(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(_lib.pv_dipole_random_state(self._h, buf, need.value, ctypes.byref(need)))
+        return buf.value.decode()
 
     # -- benchmark helpers (device-resident, no host round trip)
     def evaluateResident(self, repeats=1):

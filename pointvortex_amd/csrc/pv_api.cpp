@@ -20,6 +20,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -53,6 +54,8 @@ struct EvalRecord {
 
 struct pv_handle {
     pv_config cfg{};
+    // Dipole events: the reinjection generator, src/dipole.cpp:29-38.
+    std::mt19937_64 dipole_rng{1234567ULL ^ 0xd1b54a32d192ed03ULL};
     pv::GeoParams gp{};
     int cu_count = 256;
     hipStream_t stream = nullptr;
@@ -1091,6 +1094,239 @@ pv_status pv_dipole_candidates(pv_handle *h, double distance, uint64_t *first, u
             hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
         }
         *count = found;
+    });
+}
+
+// ---------------------------------------------------------------- dipole events
+//
+// DipoleManager::process (src/dipole.cpp:42-125) on the resident state: the candidate search
+// on the device, then on the host the order, the closest-first disjoint selection, the
+// compaction and the reinjection, with the handle's std::mt19937_64 drawn in the reference's
+// order (uniform_real_distribution<double>(0, 1), one call per draw), so removals and
+// reinjected positions are those of the reference for the same generator state.
+
+namespace {
+
+struct DipoleCandidate {
+    uint64_t first, second;
+    double d2;
+};
+
+// pi rounded to double: the same value as C++20's std::numbers::pi, which src/dipole.cpp uses.
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// All candidates of the resident state, unordered (the device search appends them).
+std::vector<DipoleCandidate> device_candidates(pv_handle *h, double distance) {
+    size_t cap = std::max<size_t>(h->cand_cap, 1024);
+    for (;;) {
+        ensure_candidates(h, cap);
+        begin_sequence(h);
+        pv::launch_dipoles(h->gp, h->cfg.geometry == PV_PERIODIC, h->cfg.geometry == PV_DISK,
+                           distance * distance, h->x, h->y, h->g, (int)h->n, h->cand_first,
+                           h->cand_second, h->cand_d2, (unsigned long long)cap, h->st,
+                           h->stream);
+        hip_check(hipGetLastError(), "launch HIP dipole search");
+        read_status(h);
+        const size_t found = (size_t)h->st_host->count;
+        if (found > cap) {
+            cap = found;
+            continue;
+        }
+        std::vector<uint64_t> a(found), b(found);
+        std::vector<double> d(found);
+        if (found) {
+            hip_check(hipMemcpyAsync(a.data(), h->cand_first, found * sizeof(uint64_t),
+                                     hipMemcpyDeviceToHost, h->stream),
+                      "read dipole candidates");
+            hip_check(hipMemcpyAsync(b.data(), h->cand_second, found * sizeof(uint64_t),
+                                     hipMemcpyDeviceToHost, h->stream),
+                      "read dipole candidates");
+            hip_check(hipMemcpyAsync(d.data(), h->cand_d2, found * sizeof(double),
+                                     hipMemcpyDeviceToHost, h->stream),
+                      "read dipole candidates");
+            hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+        }
+        std::vector<DipoleCandidate> out(found);
+        for (size_t k = 0; k < found; ++k)
+            out[k] = {a[k], b[k], d[k]};
+        return out;
+    }
+}
+
+struct HostState {
+    std::vector<double> x, y, g;
+    void push(double a, double b, double c) {
+        x.push_back(a);
+        y.push_back(b);
+        g.push_back(c);
+    }
+};
+
+double unit_draw(std::mt19937_64 &rng) {
+    std::uniform_real_distribution<double> unit(0.0, 1.0);
+    return unit(rng);
+}
+
+// src/dipole.cpp:127-133: a single vortex uniformly in the disk.
+void inject_single(const pv_config &c, std::mt19937_64 &rng, HostState &s, double g) {
+    const double radius = c.disk_radius * std::sqrt(unit_draw(rng));
+    const double angle = 2.0 * kPi * unit_draw(rng);
+    s.push(radius * std::cos(angle), radius * std::sin(angle), g);
+}
+
+// src/dipole.cpp:135-176: a pair at independent uniform positions, or (paired) a first
+// position and its partner at the mean spacing in a uniform direction.
+void inject_pair(const pv_config &c, int mode, std::mt19937_64 &rng, HostState &s, double g1,
+                 double g2, size_t population) {
+    const bool periodic = c.geometry == PV_PERIODIC;
+    const double area = periodic ? c.box_length_x * c.box_length_y
+                                 : kPi * c.disk_radius * c.disk_radius;
+    const double spacing = std::sqrt(area / (double)population);
+    auto position = [&]() {
+        if (periodic) {
+            const double px = (unit_draw(rng) - 0.5) * c.box_length_x;
+            const double py = (unit_draw(rng) - 0.5) * c.box_length_y;
+            return std::pair{px, py};
+        }
+        const double radius = c.disk_radius * std::sqrt(unit_draw(rng));
+        const double angle = 2.0 * kPi * unit_draw(rng);
+        return std::pair{radius * std::cos(angle), radius * std::sin(angle)};
+    };
+    auto first = position();
+    auto second = position();
+    if (mode == PV_REINJECT_PAIRED) {
+        bool placed = false;
+        for (size_t attempt = 0; attempt < 100000; ++attempt) {
+            first = position();
+            const double angle = 2.0 * kPi * unit_draw(rng);
+            second = {first.first + spacing * std::cos(angle),
+                      first.second + spacing * std::sin(angle)};
+            if (periodic) {
+                second.first = std::remainder(second.first, c.box_length_x);
+                second.second = std::remainder(second.second, c.box_length_y);
+                placed = true;
+                break;
+            }
+            if (second.first * second.first + second.second * second.second <
+                c.disk_radius * c.disk_radius) {
+                placed = true;
+                break;
+            }
+        }
+        if (!placed)
+            raise(PV_ERR_RUNTIME, "could not find a valid paired reinjection position");
+    }
+    s.push(first.first, first.second, g1);
+    s.push(second.first, second.second, g2);
+}
+
+} // namespace
+
+pv_status pv_dipole_set_random(pv_handle *h, uint64_t seed, const char *state) {
+    return guard([&] {
+        checked(h);
+        std::mt19937_64 rng(seed ^ 0xd1b54a32d192ed03ULL);
+        if (state && *state) {
+            std::istringstream in(state);
+            if (!(in >> rng))
+                raise(PV_ERR_RUNTIME, "invalid random-generator state in checkpoint");
+        }
+        h->dipole_rng = rng;
+    });
+}
+
+pv_status pv_dipole_random_state(pv_handle *h, char *buffer, size_t length, size_t *needed) {
+    return guard([&] {
+        checked(h);
+        std::ostringstream out;
+        out << h->dipole_rng;
+        const std::string text = out.str();
+        if (needed)
+            *needed = text.size() + 1;
+        if (buffer && length) {
+            std::strncpy(buffer, text.c_str(), length - 1);
+            buffer[length - 1] = '\0';
+        }
+    });
+}
+
+pv_status pv_dipole_process(pv_handle *h, double distance, int reinjection, size_t *events) {
+    return guard([&] {
+        checked(h);
+        require_state(h);
+        if (!events)
+            raise(PV_ERR_INVALID_ARGUMENT, "null event count");
+        if (!std::isfinite(distance) || distance < 0.0)
+            raise(PV_ERR_INVALID_ARGUMENT, "dipole removal distance must be finite and non-negative");
+        if (reinjection != PV_REINJECT_NONE && reinjection != PV_REINJECT_INDEPENDENT &&
+            reinjection != PV_REINJECT_PAIRED)
+            raise(PV_ERR_INVALID_ARGUMENT, "unknown dipole reinjection mode");
+        *events = 0;
+        const size_t n = h->n;
+        if (n == 0)
+            return;
+        DeviceGuard dg(h->cfg.device);
+        std::vector<DipoleCandidate> cand = device_candidates(h, distance);
+        if (cand.empty())
+            return;
+        // src/dipole.cpp:79-84: closest first, ties by index.
+        std::sort(cand.begin(), cand.end(), [](const DipoleCandidate &a, const DipoleCandidate &b) {
+            if (a.d2 != b.d2)
+                return a.d2 < b.d2;
+            return std::pair{a.first, a.second} < std::pair{b.first, b.second};
+        });
+        HostState cur;
+        cur.x.resize(n), cur.y.resize(n), cur.g.resize(n);
+        const size_t bytes = n * sizeof(double);
+        hip_check(hipMemcpyAsync(cur.x.data(), h->x, bytes, hipMemcpyDeviceToHost, h->stream),
+                  "download state x from GPU");
+        hip_check(hipMemcpyAsync(cur.y.data(), h->y, bytes, hipMemcpyDeviceToHost, h->stream),
+                  "download state y from GPU");
+        hip_check(hipMemcpyAsync(cur.g.data(), h->g, bytes, hipMemcpyDeviceToHost, h->stream),
+                  "download circulation from GPU");
+        hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+        // src/dipole.cpp:86-100: disjoint selection; a wall candidate (second == n) removes
+        // the real vortex only.
+        struct Event {
+            double g1, g2;
+            bool wall;
+        };
+        std::vector<char> selected(n, 0);
+        std::vector<Event> ev;
+        for (const DipoleCandidate &c : cand) {
+            if (selected[c.first] || (c.second < n && selected[c.second]))
+                continue;
+            selected[c.first] = 1;
+            if (c.second < n) {
+                selected[c.second] = 1;
+                ev.push_back({cur.g[c.first], cur.g[c.second], false});
+            } else {
+                ev.push_back({cur.g[c.first], 0.0, true});
+            }
+        }
+        // src/dipole.cpp:104-111: survivors in index order.
+        HostState next;
+        for (size_t i = 0; i < n; ++i)
+            if (!selected[i])
+                next.push(cur.x[i], cur.y[i], cur.g[i]);
+        // src/dipole.cpp:114-122.
+        if (reinjection != PV_REINJECT_NONE)
+            for (const Event &e : ev) {
+                if (e.wall)
+                    inject_single(h->cfg, h->dipole_rng, next, e.g1);
+                else
+                    inject_pair(h->cfg, reinjection, h->dipole_rng, next, e.g1, e.g2, n);
+            }
+        const size_t m = next.x.size();
+        h->fsal_valid = false;
+        h->n = m;
+        if (m) {
+            ensure_state(h, m);
+            upload_to(h, h->x, h->y, h->g, next.x.data(), next.y.data(), next.g.data(), m);
+            hip_check(hipStreamSynchronize(h->stream), "synchronize HIP stream");
+        }
+        h->state_valid = true;
+        *events = ev.size();
     });
 }
 

@@ -463,3 +463,87 @@ def test_mixed_disk_run_against_the_oracle_at_first_and_last_step(oracle):
         for b, e in [(0, 256), (n - 256, n)]:
             ur, vr = oracle.evaluate_range(spec("disk"), st.x, st.y, pop.circulation, b, e)
             assert normwise(vel.x[b:e], vel.y[b:e], ur[b:e], vr[b:e]) <= 1e-6, (b, e)
+
+
+def _planted(geometry, seed):
+    """300 vortices with four opposite-sign pairs moved to within 3e-4 of each other (and, in
+    the disk, three vortices within 2e-4 of the wall)."""
+    if geometry == "periodic":
+        pop = wl.periodic(300, seed, box_length=1.0)
+    elif geometry == "disk":
+        pop = wl.disk(300, seed, wall_fraction=0.0, gaussian=False)
+    else:
+        pop = wl.infinite(300, seed)
+    x, y, g = pop.x.copy(), pop.y.copy(), pop.circulation.copy()
+    rng = np.random.default_rng(seed)
+    used = set()
+    for i in rng.permutation(300)[:40]:
+        if len(used) >= 8 or i in used:
+            continue
+        j = next(k for k in range(300) if k not in used and k != i and g[k] == -g[i])
+        x[j], y[j] = x[i] + 3e-4 * (1 + 0.1 * len(used)), y[i]
+        used |= {i, j}
+    if geometry == "disk":
+        free = [k for k in range(300) if k not in used][:3]
+        for k, a in zip(free, (0.3, 2.0, 4.0)):
+            x[k], y[k] = (1 - 1.5e-4) * np.cos(a), (1 - 1.5e-4) * np.sin(a)
+    return x, y, g
+
+
+@pytest.mark.parametrize("geometry,mode", [("periodic", "independent"), ("periodic", "paired"),
+                                           ("disk", "paired"), ("disk", "none"),
+                                           ("infinite", "none")])
+def test_dipole_process_matches_reference(reference, geometry, mode):
+    """DipoleManager::process on the device state (device search, host selection, compaction
+    and seeded reinjection) against the reference's own, bit for bit."""
+    x, y, g = _planted(geometry, 31)
+    modes = {"none": 0, "independent": 1, "paired": 2}
+    box = 1.0 if geometry == "periodic" else 2.0
+    rx, ry, rg, rev = reference.dipole_process(geometry, box, 1.0, 1e-3, modes[mode], 42, x, y, g)
+    k = kernel_for(geometry, boxLengthX=box, boxLengthY=box)
+    k.uploadDeviceState(VortexSystem(x, y, g))
+    k.setDipoleRandom(42)
+    ev = k.processDipoles(1e-3, mode)
+    assert ev == rev and ev >= 4
+    out = k.downloadDeviceSystem()
+    assert out.size() == len(rx)
+    assert np.array_equal(out.x, rx) and np.array_equal(out.y, ry)
+    assert np.array_equal(out.circulation, rg)
+
+
+def test_dipole_generator_state_round_trip(reference):
+    """The generator's text state (as checkpoints store it) continues the sequence: a second
+    handle restored from it reinjects exactly as the first handle does next."""
+    x, y, g = _planted("periodic", 32)
+    k = kernel_for("periodic", boxLengthX=1.0, boxLengthY=1.0)
+    k.setDipoleRandom(7)
+    k.uploadDeviceState(VortexSystem(x, y, g))
+    k.processDipoles(1e-3, "independent")
+    state = k.dipoleRandomState()
+    x2, y2, g2 = _planted("periodic", 33)
+    k.uploadDeviceState(VortexSystem(x2, y2, g2))
+    k.processDipoles(1e-3, "independent")
+    a = k.downloadDeviceSystem()
+    q = kernel_for("periodic", boxLengthX=1.0, boxLengthY=1.0)
+    q.setDipoleRandom(7, state)
+    q.uploadDeviceState(VortexSystem(x2, y2, g2))
+    q.processDipoles(1e-3, "independent")
+    b = q.downloadDeviceSystem()
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    assert np.array_equal(a.circulation, b.circulation)
+    with pytest.raises(BackendRuntimeError, match="random-generator state"):
+        q.setDipoleRandom(7, "not a generator state")
+
+
+def test_dipole_process_then_evaluate(oracle):
+    """After an event the resident state is the new one (the FSAL derivative is gone): the
+    next evaluation matches the oracle on the compacted, reinjected population."""
+    x, y, g = _planted("disk", 34)
+    k = kernel_for("disk")
+    k.uploadDeviceState(VortexSystem(x, y, g))
+    k.setDipoleRandom(3)
+    assert k.processDipoles(1e-3, "paired") >= 4
+    s = k.downloadDeviceSystem()
+    vel = k.evaluateDeviceState()
+    ur, vr = oracle.evaluate(spec("disk"), s.x, s.y, s.circulation)
+    assert normwise(vel.x, vel.y, ur, vr) <= fp64_gate("disk", s.x, s.y, s.circulation, ur, vr)
