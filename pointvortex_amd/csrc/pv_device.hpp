@@ -44,13 +44,16 @@ struct GeoParams {
 
 // ---------------------------------------------------------------- device math helpers
 
-// FP64 reciprocal: hardware v_rcp_f64 seed refined by two Newton steps (error ~1 ulp).
+// FP64 reciprocal: the hardware v_rcp_f64 seed r (relative error |e| <= 2^-24.4 on gfx950,
+// measured over 40 binades) refined by one third-order step of the series
+// 1/a = r / (1 - e) = r (1 + e + e^2 + ...), e = 1 - a r:  r' = r + r (e + e^2).  The
+// truncation is e^3 <= 2^-73; e is formed by one FMA (relative error 2^-53), so r' is within
+// about 1 ulp, as two Newton steps are, with one FMA fewer.  Infinite or zero arguments give
+// NaN, as the Newton form does (e = -inf * 0).
 __device__ __forceinline__ double rcp64(double a) {
-    double r = __builtin_amdgcn_rcp(a);
-    double e = __builtin_fma(-a, r, 1.0);
-    r = __builtin_fma(e, r, r);
-    e = __builtin_fma(-a, r, 1.0);
-    return __builtin_fma(e, r, r);
+    const double r = __builtin_amdgcn_rcp(a);
+    const double e = __builtin_fma(-a, r, 1.0);
+    return __builtin_fma(r, __builtin_fma(e, e, e), r);
 }
 
 __device__ __forceinline__ float rcp32(float a) { return __builtin_amdgcn_rcpf(a); }
@@ -512,7 +515,7 @@ struct DiskF64 {
         double x, y, ix, iy, r2;
     };
     struct SymTmp {
-        double dx, dy, ex, ey, fx, fy, rho;    // x_i - x_j, x_i - x_j*, x_j - x_i*, |x_i|^2 w_j
+        double dx, dy, ex, ey, fx, fy, r2, w;  // x_i - x_j, x_i - x_j*, x_j - x_i*, |x_i|^2, w_j
     };
     __device__ static double sym_g(const GeoParams &, double g) { return kInvTwoPi * g; }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) {
@@ -529,7 +532,8 @@ struct DiskF64 {
         d[1] = __builtin_fma(m.ex, m.ex, m.ey * m.ey);
         m.fx = s.x - t.ix;
         m.fy = s.y - t.iy;
-        m.rho = t.r2 * s.w;
+        m.r2 = t.r2;
+        m.w = s.w;
     }
     __device__ static void sym_apply(const Src &s, double gi, const SymTmp &m, const double *inv,
                                      Acc &a, Acc &r) {
@@ -542,7 +546,8 @@ struct DiskF64 {
         const double ci = gi * inv[0];
         r.u = __builtin_fma(ci, m.dy, r.u);
         r.v = __builtin_fma(-ci, m.dx, r.v);
-        const double ki = gi * (inv[1] * m.rho);   // i's image has strength -G_i
+        // i's image has strength -G_i; G_i |x_i|^2 is the same for every source of the loop.
+        const double ki = (gi * m.r2) * (inv[1] * m.w);
         r.u = __builtin_fma(ki, m.fy, r.u);
         r.v = __builtin_fma(-ki, m.fx, r.v);
     }

@@ -52,7 +52,7 @@ struct PairLaunch {
     StageOp op;
 };
 
-// Symmetric evaluation of a whole population (FP64 policies with kSym): one block per
+// Symmetric evaluation of a whole population (every geometry, FP64 and mixed): one block per
 // unordered pair of tiles (I <= J) of kBlock*TPT vortices.  Off-diagonal blocks compute each
 // pair once and write the target side (into tile I's vortices, slot J) and the reaction (into
 // tile J's vortices, slot I); diagonal blocks compute the ordered pairs inside their tile (slot
