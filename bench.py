@@ -275,8 +275,9 @@ def c1_whole(pv, steps=1000, dt=1e-3):
     pop = wl.config1()
     n = pop.n
     out = {"n": n, "steps": steps, "dt": dt}
-    for prec in ("fp64", "mixed"):
+    def run(prec, persistent):
         k = pv.HipKernel(pv.SimParams(boundaryCondition="infinite", precision=prec))
+        k.set_persistent(persistent, 0, 0.0)
         s = pv.VortexSystem(pop.x, pop.y, pop.circulation)
         k.uploadDeviceState(s)
         k.deviceRk4Advance(dt, 10)
@@ -286,11 +287,23 @@ def c1_whole(pv, steps=1000, dt=1e-3):
         k.deviceRk4Advance(dt, steps)
         k.synchronize()
         wall = time.perf_counter() - t0
-        out[prec] = {"wall_s": wall, "pairs_per_s": 4 * steps * n * (n - 1) / wall,
-                     "launch_shape": list(k.resolved_launch(n, n))}
+        blocks = k.persistent_blocks(n)
+        shape = list(k.resolved_launch(n, n))
         k.close()
-    log(f"[bench] c1 whole: fp64 {out['fp64']['wall_s']:.3f} s, "
-        f"mixed {out['mixed']['wall_s']:.3f} s for {steps} steps")
+        return wall, blocks, shape
+
+    for prec in ("fp64", "mixed"):
+        # The default path (the persistent RK4 kernel at this N), then the per-stage launches.
+        wall, blocks, _ = run(prec, -1)
+        staged, _, shape = run(prec, 0)
+        out[prec] = {"wall_s": wall, "pairs_per_s": 4 * steps * n * (n - 1) / wall,
+                     "persistent_blocks": blocks,
+                     "per_stage_launches": {"wall_s": staged, "launch_shape": shape,
+                                            "pairs_per_s": 4 * steps * n * (n - 1) / staged}}
+    log(f"[bench] c1 whole: fp64 {out['fp64']['wall_s']:.4f} s "
+        f"(per-stage launches {out['fp64']['per_stage_launches']['wall_s']:.4f} s), "
+        f"mixed {out['mixed']['wall_s']:.4f} s "
+        f"(per-stage {out['mixed']['per_stage_launches']['wall_s']:.4f} s) for {steps} steps")
     return out
 
 

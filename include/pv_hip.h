@@ -208,6 +208,22 @@ pv_status pv_set_launch_table(pv_handle *h, const pv_launch_rule *rules, size_t 
 pv_status pv_resolve_launch(pv_handle *h, size_t targets, size_t n, int *targets_per_thread,
                             int *source_splits, int *tile, int *symmetric);
 
+/* Persistent RK4 path for small populations (no reference counterpart; it serves
+ * deviceRk4Step, src/compute.h:33, and pv_rk4_advance).  pv_rk4_step and pv_rk4_advance then
+ * run all their stages in one cooperative launch: each block owns targets_per_block targets
+ * (4, 8 or 16; 0 = 8) for the whole run and the blocks exchange stage positions through HBM
+ * and a grid barrier, instead of one launch per stage.  Results agree with the per-stage
+ * launches to rounding (each target's sum runs in another order); failures, their messages
+ * and the state kept after one are the same.  mode -1 (default): used for N <= 2048 unless a
+ * launch shape was forced with pv_set_launch / pv_set_launch_shape; 1: used whenever every
+ * block fits on the device at once (up to N = 65536); 0: never.  A barrier wait longer than
+ * max_wait_seconds (0 = 2 s) ends the launch with PV_ERR_RUNTIME instead of waiting on.
+ * pv_persistent_blocks gives the blocks a step of n vortices would launch (0: per-stage
+ * launches). */
+pv_status pv_set_persistent(pv_handle *h, int mode, int targets_per_block,
+                            double max_wait_seconds);
+pv_status pv_persistent_blocks(pv_handle *h, size_t n, int *blocks);
+
 /* Kernel timing for benchmarks (no reference counterpart): between begin and end every
  * velocity-kernel launch of the handle is bracketed by HIP events recorded on the handle's own
  * stream; end waits and returns the summed kernel time and the number of launches. */

@@ -156,6 +156,8 @@ _SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int)]),
+    "pv_set_persistent": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    "pv_persistent_blocks": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]),
     "pv_profile_begin": (ctypes.c_int, [_vp]),
     "pv_profile_end": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "pv_calibration_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp]),
@@ -330,6 +332,19 @@ class HipKernel:
         not forced, 0 = split path, 1 = symmetric path for whole-population evaluations)."""
         _check(_lib.pv_set_launch_shape(self._h, int(targets_per_thread), int(source_splits),
                                         int(tile), int(symmetric)))
+
+    def set_persistent(self, mode: int = -1, targets_per_block: int = 0,
+                      max_wait_seconds: float = 0.0):
+        """RK4 steps in one cooperative launch for small populations: mode -1 automatic
+        (N <= 2048 unless a launch shape was forced), 1 whenever it fits, 0 never."""
+        _check(_lib.pv_set_persistent(self._h, int(mode), int(targets_per_block),
+                                      float(max_wait_seconds)))
+
+    def persistent_blocks(self, n: int) -> int:
+        """Blocks of the persistent RK4 launch for n vortices (0: per-stage launches)."""
+        b = ctypes.c_int()
+        _check(_lib.pv_persistent_blocks(self._h, int(n), ctypes.byref(b)))
+        return b.value
 
     def set_launch_table(self, rules):
         """rules: iterable of (geometry, precision, n_max, targets_per_thread, source_splits,

@@ -94,6 +94,14 @@ struct pv_handle {
     unsigned epoch_base = 1;
     std::vector<EvalRecord> records;
 
+    // Persistent RK4 path (pv_persist.hip): mode -1 automatic, 0 never, 1 whenever it fits.
+    int persist_mode = -1, persist_tb = 0;
+    double persist_wait_s = 2.0;     // bound on one grid-barrier wait
+    bool coop = false;               // the device takes cooperative launches
+    double wall_khz = 100000.0;      // rate of the device's constant wall clock
+    int persist_cap[2][3] = {{-1, -1, -1}, {-1, -1, -1}};   // co-resident blocks, -1 unknown
+    unsigned long long barrier_arrivals = 0;   // since the status word was last reset
+
     ~pv_handle() {
         if (stream)
             (void)hipStreamSynchronize(stream);
@@ -217,6 +225,7 @@ void begin_sequence(pv_handle *h) {
     hip_check(hipGetLastError(), "launch HIP status reset");
     h->epoch_base = h->epoch + 1;
     h->records.clear();
+    h->barrier_arrivals = 0;
 }
 
 struct Shape {
@@ -493,6 +502,8 @@ void finish(pv_handle *h, bool velocity_must_be_finite) {
     }
     if (flags & pv::kFailErrorEstimate)
         raise(PV_ERR_RUNTIME, "non-finite adaptive error estimate or invalid tolerance");
+    if (flags & pv::kFailBarrier)
+        raise(PV_ERR_RUNTIME, "grid barrier of the persistent RK4 kernel timed out");
 }
 
 // src/vortex.h:7-14, src/compute.cpp:31-48 on host arrays.
@@ -633,6 +644,14 @@ pv_status pv_create(const pv_config *config, pv_handle **out) {
         hipDeviceProp_t prop{};
         hip_check(hipGetDeviceProperties(&prop, config->device), "query HIP device properties");
         h->cu_count = std::max(1, prop.multiProcessorCount);
+        int coop = 0, wall = 0;
+        if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, config->device) ==
+            hipSuccess)
+            h->coop = coop != 0;
+        if (hipDeviceGetAttribute(&wall, hipDeviceAttributeWallClockRate, config->device) ==
+                hipSuccess &&
+            wall > 0)
+            h->wall_khz = wall;
         hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "create HIP stream");
         h->st = dalloc<pv::DevStatus>(1, "hipMalloc(status)");
         hip_check(hipHostMalloc((void **)&h->st_host, sizeof(pv::DevStatus), hipHostMallocDefault),
@@ -693,6 +712,35 @@ pv_status pv_set_launch_shape(pv_handle *h, int targets_per_thread, int source_s
         h->split_override = source_splits;
         h->tile_override = tile;
         h->sym_override = symmetric;
+    });
+}
+
+static int persistent_blocks(pv_handle *h, size_t n);
+
+pv_status pv_set_persistent(pv_handle *h, int mode, int targets_per_block,
+                            double max_wait_seconds) {
+    return guard([&] {
+        checked(h);
+        if (mode < -1 || mode > 1)
+            raise(PV_ERR_INVALID_ARGUMENT, "persistent mode must be -1, 0 or 1");
+        if (targets_per_block != 0 && targets_per_block != 8 && targets_per_block != 16 &&
+            targets_per_block != 32)
+            raise(PV_ERR_INVALID_ARGUMENT, "targets per block must be 0, 8, 16 or 32");
+        if (!std::isfinite(max_wait_seconds) || max_wait_seconds < 0.0)
+            raise(PV_ERR_INVALID_ARGUMENT, "barrier wait must be finite and non-negative");
+        h->persist_mode = mode;
+        h->persist_tb = targets_per_block;
+        h->persist_wait_s = max_wait_seconds > 0.0 ? max_wait_seconds : 2.0;
+    });
+}
+
+pv_status pv_persistent_blocks(pv_handle *h, size_t n, int *blocks) {
+    return guard([&] {
+        checked(h);
+        if (!blocks)
+            raise(PV_ERR_INVALID_ARGUMENT, "null output");
+        DeviceGuard dg(h->cfg.device);
+        *blocks = persistent_blocks(h, n);
     });
 }
 
@@ -877,9 +925,15 @@ pv_status pv_evaluate_device_state(pv_handle *h, double *u, double *v, size_t n)
 }
 
 // Enqueues one RK4 step reading (x0, y0) and writing (xn, yn) (src/timestep.cpp:31-61).
-static void enqueue_rk4(pv_handle *h, double dt, const double *x0, const double *y0, double *xn,
-                        double *yn) {
-    const size_t n = h->n;
+// The four stage evaluations of one RK4 step (src/timestep.cpp:36-58): the positions each
+// evaluates and its fused epilogue.
+struct Rk4Stage {
+    const double *x, *y;
+    pv::StageOp op;
+};
+
+static void rk4_stages(pv_handle *h, double dt, const double *x0, const double *y0, double *xn,
+                       double *yn, Rk4Stage st[4]) {
     pv::StageOp op{};
     op.dt = dt;
     op.x0 = x0, op.y0 = y0;
@@ -889,19 +943,101 @@ static void enqueue_rk4(pv_handle *h, double dt, const double *x0, const double 
     op.kind = pv::kOpStage, op.cur = 0;
     op.coef[0] = 0.5;
     op.ku = h->kx[0], op.kv = h->ky[0], op.xn = h->xa, op.yn = h->ya;
-    evaluate(h, x0, y0, h->g, n, 0, n, op);
+    st[0] = {x0, y0, op};
     // k1 -> stage 2 position x0 + dt (0 k0 + 1/2 k1)
     op.cur = 1, op.coef[0] = 0.0, op.coef[1] = 0.5;
     op.ku = h->kx[1], op.kv = h->ky[1], op.xn = h->xb, op.yn = h->yb;
-    evaluate(h, h->xa, h->ya, h->g, n, 0, n, op);
+    st[1] = {h->xa, h->ya, op};
     // k2 -> stage 3 position x0 + dt (0 k0 + 0 k1 + 1 k2)
     op.cur = 2, op.coef[1] = 0.0, op.coef[2] = 1.0;
     op.ku = h->kx[2], op.kv = h->ky[2], op.xn = h->xa, op.yn = h->ya;
-    evaluate(h, h->xb, h->yb, h->g, n, 0, n, op);
+    st[2] = {h->xb, h->yb, op};
     // k3 -> x0 + dt (k0 + 2 k1 + 2 k2 + k3) / 6
     op.kind = pv::kOpRk4Final, op.cur = 3;
     op.ku = h->kx[3], op.kv = h->ky[3], op.xn = xn, op.yn = yn;
-    evaluate(h, h->xa, h->ya, h->g, n, 0, n, op);
+    st[3] = {h->xa, h->ya, op};
+}
+
+static void enqueue_rk4(pv_handle *h, double dt, const double *x0, const double *y0, double *xn,
+                        double *yn) {
+    Rk4Stage st[4];
+    rk4_stages(h, dt, x0, y0, xn, yn, st);
+    for (const Rk4Stage &s : st)
+        evaluate(h, s.x, s.y, h->g, h->n, 0, h->n, s.op);
+}
+
+// Persistent RK4 path: beyond this population the per-stage launches run (automatic mode),
+// and nothing larger is ever launched persistently.
+constexpr size_t kPersistAutoN = 2048;
+constexpr size_t kPersistMaxN = 65536;
+constexpr int kPersistTb = 16;
+
+// Blocks of the persistent RK4 launch for n vortices, or 0 when the per-stage launches run:
+// automatic mode takes it for n <= kPersistAutoN unless a launch shape was forced; both modes
+// need cooperative launches and every block resident at once.
+static int persistent_blocks(pv_handle *h, size_t n) {
+    if (h->persist_mode == 0 || !h->coop || n == 0 || n > kPersistMaxN)
+        return 0;
+    if (h->persist_mode < 0 &&
+        (n > kPersistAutoN || h->tpt_override || h->split_override || h->tile_override ||
+         h->sym_override >= 0))
+        return 0;
+    const int tb = h->persist_tb ? h->persist_tb : kPersistTb;
+    const int blocks = (int)((n + tb - 1) / tb);
+    int &cap = h->persist_cap[h->cfg.precision][h->cfg.geometry];
+    if (cap < 0)
+        cap = pv::persistent_capacity(h->cfg.geometry, h->cfg.precision, h->cu_count);
+    return blocks <= cap ? blocks : 0;
+}
+
+// `steps` RK4 steps in one cooperative launch; step s reads (bx[s & 1], by[s & 1]) and writes
+// the other pair, as the per-stage launches of pv_rk4_advance do, and every stage keeps the
+// epoch and evaluation record it would have there.  Returns false, with nothing enqueued and
+// the handle unchanged, when the runtime refuses the cooperative launch (only one process can
+// hold the device's cooperative resources at a time); the handle then keeps to the per-stage
+// launches.
+static bool enqueue_rk4_persistent(pv_handle *h, double dt, int steps, int blocks, double *bx[2],
+                                   double *by[2]) {
+    const size_t n = h->n;
+    pv::RkPersistLaunch a{};
+    Rk4Stage st[2][4];
+    for (int p = 0; p < 2; ++p) {
+        rk4_stages(h, dt, bx[p], by[p], bx[1 - p], by[1 - p], st[p]);
+        a.x[p] = bx[p], a.y[p] = by[p];
+        for (int c = 0; c < 4; ++c)
+            a.ops[4 * p + c] = st[p][c].op;
+    }
+    a.g = h->g;
+    a.xa = h->xa, a.ya = h->ya, a.xb = h->xb, a.yb = h->yb;
+    a.n = (int)n;
+    a.steps = steps;
+    a.tb = h->persist_tb ? h->persist_tb : kPersistTb;
+    a.st = h->st;
+    a.epoch0 = h->epoch + 1;
+    a.barrier_base = h->barrier_arrivals;
+    a.wait_limit = (unsigned long long)(h->persist_wait_s * h->wall_khz * 1e3);
+    const size_t events_used = h->events_used;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    record_events(h, ev0, ev1);
+    if (ev0)
+        hip_check(hipEventRecord(ev0, h->stream), "record HIP event");
+    const hipError_t e = pv::launch_rk4_persistent(h->cfg.geometry, h->cfg.precision, a, h->gp,
+                                                   blocks, h->stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        h->coop = false;
+        h->events_used = events_used;
+        return false;
+    }
+    if (ev1)
+        hip_check(hipEventRecord(ev1, h->stream), "record HIP event");
+    h->epoch += 4u * (unsigned)steps;
+    h->barrier_arrivals += (unsigned long long)blocks * (4ull * (unsigned)steps - 1ull);
+    h->records.reserve(h->records.size() + 4 * (size_t)steps);
+    for (int s = 0; s < steps; ++s)
+        for (int c = 0; c < 4; ++c)
+            h->records.push_back({st[s & 1][c].x, st[s & 1][c].y, (int)n, 0, (int)n});
+    return true;
 }
 
 pv_status pv_rk4_step(pv_handle *h, double dt) {
@@ -914,7 +1050,10 @@ pv_status pv_rk4_step(pv_handle *h, double dt) {
         DeviceGuard dg(h->cfg.device);
         h->fsal_valid = false;
         begin_sequence(h);
-        enqueue_rk4(h, dt, h->x, h->y, h->xalt, h->yalt);
+        double *bx[2] = {h->x, h->xalt}, *by[2] = {h->y, h->yalt};
+        const int blocks = persistent_blocks(h, h->n);
+        if (!blocks || !enqueue_rk4_persistent(h, dt, 1, blocks, bx, by))
+            enqueue_rk4(h, dt, h->x, h->y, h->xalt, h->yalt);
         finish(h, true);
         std::swap(h->x, h->xalt);
         std::swap(h->y, h->yalt);
@@ -934,8 +1073,11 @@ pv_status pv_rk4_advance(pv_handle *h, double dt, int steps) {
         h->fsal_valid = false;
         begin_sequence(h);
         double *bx[2] = {h->x, h->xalt}, *by[2] = {h->y, h->yalt};
-        for (int s = 0; s < steps; ++s)
-            enqueue_rk4(h, dt, bx[s & 1], by[s & 1], bx[(s + 1) & 1], by[(s + 1) & 1]);
+        const int blocks = persistent_blocks(h, h->n);
+        if (!blocks || !enqueue_rk4_persistent(h, dt, steps, blocks, bx, by)) {
+            for (int s = 0; s < steps; ++s)
+                enqueue_rk4(h, dt, bx[s & 1], by[s & 1], bx[(s + 1) & 1], by[(s + 1) & 1]);
+        }
         try {
             finish(h, true);
         } catch (ApiError &e) {

@@ -426,6 +426,7 @@ __global__ void reset_status_kernel(DevStatus *st) {
         st->fail_epoch = UINT_MAX;
         st->err_bits = 0ull;
         st->count = 0ull;
+        st->barrier = 0ull;
     }
 }
 

@@ -12,7 +12,8 @@ struct DevStatus {
     unsigned fail_epoch;           // epoch of the first failing launch (UINT_MAX: none)
     unsigned long long err_bits;   // DOPRI5 max-norm error (bit pattern of a double >= 0)
     unsigned long long count;      // dipole candidates found
-    unsigned long long pad[4];
+    unsigned long long barrier;    // grid-barrier arrivals of the persistent RK4 kernel
+    unsigned long long pad[3];
 };
 
 enum FailureKind : unsigned {
@@ -21,6 +22,7 @@ enum FailureKind : unsigned {
     kFailVelocity = 4u,          // non-finite velocity (coincident pairs end up here too)
     kFailErrorEstimate = 8u,     // "non-finite adaptive error estimate or invalid tolerance"
     kFailCoincident = 16u,       // set by the classification pass
+    kFailBarrier = 32u,          // a grid barrier of the persistent RK4 kernel timed out
 };
 
 enum OpKind : int { kOpStore = 0, kOpStage = 1, kOpRk4Final = 2, kOpDopriError = 3 };
@@ -72,6 +74,27 @@ constexpr int kBlock = 256;
 constexpr int kTileF64 = 256;
 constexpr int kTileMixed = 128;
 
+// RK4 steps of a whole population in one cooperative launch (pv_persist.hip): block b of
+// kPersistBlock threads owns targets [b tb, b tb + tb), tb in {8, 16, 32}; its
+// kPersistBlock / tb lane groups split the sources, staged in LDS kPersistChunk at a time.
+// ops[4 p + c] is stage c of a step of parity p (p = step & 1), exactly as the per-stage
+// launches would be given it; step s evaluates stage 0 at (x[p], y[p]) and writes the new
+// state to (x[1-p], y[1-p]).
+constexpr int kPersistBlock = 1024;
+constexpr int kPersistChunk = 1024;
+constexpr int kPersistMaxTb = 32;
+struct RkPersistLaunch {
+    const double *x[2], *y[2];
+    const double *g;
+    const double *xa, *ya, *xb, *yb;   // stage positions (written through ops[].xn)
+    int n, steps, tb;
+    DevStatus *st;
+    unsigned epoch0;                   // epoch of stage 0 of step 0; stage c of step s: +4s+c
+    unsigned long long barrier_base;   // arrivals counted before this launch
+    unsigned long long wait_limit;     // barrier wait bound, in wall-clock ticks
+    StageOp ops[8];
+};
+
 // Launchers (pv_kernels.hip).  `tpt` is 1, 2 or 4.
 // tile: sources staged in LDS per step, 128 or 256 (pair_tile() gives each precision's default).
 void launch_pair(int geometry, int precision, int tpt, int tile, const PairLaunch &a,
@@ -98,6 +121,11 @@ void launch_dipoles(const GeoParams &gp, bool periodic, bool disk, double thr2, 
                     DevStatus *st, hipStream_t s);
 
 void launch_copy(const double4 *src, double4 *dst, size_t count, hipStream_t s);
+
+// Blocks of the persistent RK4 kernel that fit on the device at once (0 if unknown).
+int persistent_capacity(int geometry, int precision, int cu_count);
+hipError_t launch_rk4_persistent(int geometry, int precision, const RkPersistLaunch &a,
+                                 const GeoParams &gp, int blocks, hipStream_t s);
 
 constexpr int kReduceBlock = 256;
 
