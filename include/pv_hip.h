@@ -211,12 +211,14 @@ pv_status pv_resolve_launch(pv_handle *h, size_t targets, size_t n, int *targets
 /* Persistent RK4 path for small populations (no reference counterpart; it serves
  * deviceRk4Step, src/compute.h:33, and pv_rk4_advance).  pv_rk4_step and pv_rk4_advance then
  * run all their stages in one cooperative launch: each block owns targets_per_block targets
- * (4, 8 or 16; 0 = 8) for the whole run and the blocks exchange stage positions through HBM
+ * (8, 16 or 32; 0 = 16) for the whole run and the blocks exchange stage positions through HBM
  * and a grid barrier, instead of one launch per stage.  Results agree with the per-stage
  * launches to rounding (each target's sum runs in another order); failures, their messages
- * and the state kept after one are the same.  mode -1 (default): used for N <= 2048 unless a
+ * and the state kept after one are the same.  mode -1 (default): used for N <= 4096 unless a
  * launch shape was forced with pv_set_launch / pv_set_launch_shape; 1: used whenever every
- * block fits on the device at once (up to N = 65536); 0: never.  A barrier wait longer than
+ * block fits on the device at once (up to N = 65536); 0: never.  When the runtime refuses the
+ * cooperative launch (another process holds the device's cooperative resources) the handle
+ * keeps to the per-stage launches.  A barrier wait longer than
  * max_wait_seconds (0 = 2 s) ends the launch with PV_ERR_RUNTIME instead of waiting on.
  * pv_persistent_blocks gives the blocks a step of n vortices would launch (0: per-stage
  * launches). */

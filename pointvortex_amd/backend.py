@@ -336,7 +336,8 @@ class HipKernel:
     def set_persistent(self, mode: int = -1, targets_per_block: int = 0,
                       max_wait_seconds: float = 0.0):
         """RK4 steps in one cooperative launch for small populations: mode -1 automatic
-        (N <= 2048 unless a launch shape was forced), 1 whenever it fits, 0 never."""
+        (N <= 4096 unless a launch shape was forced), 1 whenever it fits, 0 never;
+        targets_per_block 8, 16 or 32 (0 = 16)."""
         _check(_lib.pv_set_persistent(self._h, int(mode), int(targets_per_block),
                                       float(max_wait_seconds)))
 

@@ -968,7 +968,7 @@ static void enqueue_rk4(pv_handle *h, double dt, const double *x0, const double 
 
 // Persistent RK4 path: beyond this population the per-stage launches run (automatic mode),
 // and nothing larger is ever launched persistently.
-constexpr size_t kPersistAutoN = 2048;
+constexpr size_t kPersistAutoN = 4096;
 constexpr size_t kPersistMaxN = 65536;
 constexpr int kPersistTb = 16;
 

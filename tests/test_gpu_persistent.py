@@ -69,10 +69,10 @@ def displacement_error(a, b, pop):
 def test_when_the_persistent_path_is_taken():
     k = kernel("infinite")
     assert k.persistent_blocks(1024) == 64           # 16 targets per block
-    assert k.persistent_blocks(2048) == 128
-    assert k.persistent_blocks(4096) == 0            # automatic mode stops at 2048
+    assert k.persistent_blocks(4096) == 256
+    assert k.persistent_blocks(4097) == 0            # automatic mode stops at 4096
     k.set_persistent(1, 32, 0.0)
-    assert k.persistent_blocks(4096) == 128
+    assert k.persistent_blocks(8192) == 256
     assert k.persistent_blocks(65537) == 0
     k.set_persistent(0, 0, 0.0)
     assert k.persistent_blocks(1024) == 0
