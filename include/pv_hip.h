@@ -307,13 +307,16 @@ pv_status pv_batch_failures(pv_batch *b, unsigned *failures, unsigned long long 
 /* Adaptive ensembles with admission while others run (serving).  pv_batch_admit places one
  * run (DOPRI5 from t=0 to t_end) if its vortices fit; *slot is -1 when they do not.
  * pv_batch_attempt performs one DOPRI5 attempt for every running instance with the
- * step-size controllers on the device; pv_batch_poll reads every occupied slot's state;
+ * step-size controllers on the device; pv_batch_attempts performs `count` of them back to back
+ * with one synchronisation at the end (a run that finishes or fails part-way sits out the
+ * rest); pv_batch_poll reads every occupied slot's state as of the last attempt;
  * pv_batch_fetch copies a slot's positions; pv_batch_release frees the slot. */
 pv_status pv_batch_admit(pv_batch *b, const pv_instance_config *config, const double *x,
                          const double *y, const double *gamma, size_t n, double t_end,
                          double dt0, double atol, double rtol, double min_dt, double max_dt,
                          int *slot);
 pv_status pv_batch_attempt(pv_batch *b);
+pv_status pv_batch_attempts(pv_batch *b, int count);
 pv_status pv_batch_poll(pv_batch *b, pv_slot_state *states, size_t cap, size_t *count);
 pv_status pv_batch_fetch(pv_batch *b, int slot, double *x, double *y, size_t n);
 pv_status pv_batch_release(pv_batch *b, int slot);

@@ -185,6 +185,7 @@ _SIGNATURES = {
                                       ctypes.c_size_t] + [ctypes.c_double] * 6 +
                        [ctypes.POINTER(ctypes.c_int)]),
     "pv_batch_attempt": (ctypes.c_int, [_vp]),
+    "pv_batch_attempts": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pv_batch_poll": (ctypes.c_int, [_vp, ctypes.POINTER(PvSlotState), ctypes.c_size_t, _szp]),
     "pv_batch_fetch": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_size_t]),
     "pv_batch_release": (ctypes.c_int, [_vp, ctypes.c_int]),

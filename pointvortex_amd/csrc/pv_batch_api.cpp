@@ -809,68 +809,78 @@ pv_status pv_batch_admit(pv_batch *b, const pv_instance_config *config, const do
     });
 }
 
-pv_status pv_batch_attempt(pv_batch *b) {
+// One DOPRI5 attempt of every running run, enqueued on the ensemble's stream.
+static void enqueue_attempt(pv_batch *b) {
+    if (b->items_dirty) {
+        std::vector<int> running;
+        b->in_items.assign(b->max_inst, 0);
+        for (size_t k = 0; k < b->max_inst; ++k)
+            if (b->used[k] && b->steps_h[k].state == pv::kRunning) {
+                running.push_back((int)k);
+                b->in_items[k] = 1;
+            }
+        build_items(b, running);
+        b->items_dirty = false;
+    }
+    if (b->nitems == 0)
+        return;
+    // Per-vortex passes only need to reach the end of the highest occupied range.
+    size_t high = 0;
+    for (size_t k = 0; k < b->max_inst; ++k)
+        if (b->used[k])
+            high = std::max(high, (size_t)b->inst_h[k].offset + (size_t)b->inst_h[k].n);
+    const int n = (int)high;
+    b->span = n;
+    pv::StageOp op{};
+    for (int s = 0; s < 7; ++s) {
+        op.kx[s] = b->kx[s], op.ky[s] = b->ky[s];
+        op.w[s] = kB5[s] - kB4[s];
+    }
+    op.x0 = b->x, op.y0 = b->y;
+    // FSAL derivative for runs that have none yet (src/timestep.cpp:111-112).
+    op.kind = pv::kOpStore;
+    op.ku = b->kx[0], op.kv = b->ky[0];
+    launch(b, b->x, b->y, op, true, pv::kPassNeedK0);
+    // Stage positions: 1 -> A, 2 -> B, 3 -> A, 4 -> B, 5 -> A, 6 (fifth order) -> next.
+    double *px[7] = {nullptr, b->xa, b->xb, b->xa, b->xb, b->xa, b->xn};
+    double *py[7] = {nullptr, b->ya, b->yb, b->ya, b->yb, b->ya, b->yn};
+    op.kind = pv::kOpStage, op.cur = 0, op.coef[0] = kDopriA[1][0];
+    op.xn = px[1], op.yn = py[1];
+    pv::launch_batch_stage1(op, b->owner, b->steps_d, n, b->stream);
+    hip_check(hipGetLastError(), "launch HIP ensemble stage");
+    for (int s = 1; s < 7; ++s) {
+        op.cur = s;
+        op.ku = b->kx[s], op.kv = b->ky[s];
+        if (s < 6) {
+            op.kind = pv::kOpStage;
+            for (int j = 0; j <= s; ++j)
+                op.coef[j] = kDopriA[s + 1][j];
+            op.xn = px[s + 1], op.yn = py[s + 1];
+        } else {
+            op.kind = pv::kOpDopriError;
+            op.xn = nullptr, op.yn = nullptr;
+        }
+        launch(b, px[s], py[s], op, true, pv::kPassAll);
+    }
+    pv::launch_batch_control(b->steps_d, (int)b->max_inst, b->stream);
+    hip_check(hipGetLastError(), "launch HIP ensemble controller");
+    pv::launch_batch_accept(b->owner, b->steps_d, b->x, b->y, b->xn, b->yn, b->kx[0],
+                            b->ky[0], b->kx[6], b->ky[6], n, b->stream);
+    hip_check(hipGetLastError(), "launch HIP ensemble accept");
+}
+
+pv_status pv_batch_attempts(pv_batch *b, int count) {
     return guard([&] {
         checked(b);
         if (!b->adaptive)
             raise(PV_ERR_LOGIC, "no admitted runs in this ensemble");
+        if (count < 1)
+            raise(PV_ERR_INVALID_ARGUMENT, "attempt count must be positive");
         DeviceGuard dg(b->device);
-        if (b->items_dirty) {
-            std::vector<int> running;
-            b->in_items.assign(b->max_inst, 0);
-            for (size_t k = 0; k < b->max_inst; ++k)
-                if (b->used[k] && b->steps_h[k].state == pv::kRunning) {
-                    running.push_back((int)k);
-                    b->in_items[k] = 1;
-                }
-            build_items(b, running);
-            b->items_dirty = false;
-        }
-        if (b->nitems == 0)
-            return;
-        // Per-vortex passes only need to reach the end of the highest occupied range.
-        size_t high = 0;
-        for (size_t k = 0; k < b->max_inst; ++k)
-            if (b->used[k])
-                high = std::max(high, (size_t)b->inst_h[k].offset + (size_t)b->inst_h[k].n);
-        const int n = (int)high;
-        b->span = n;
-        pv::StageOp op{};
-        for (int s = 0; s < 7; ++s) {
-            op.kx[s] = b->kx[s], op.ky[s] = b->ky[s];
-            op.w[s] = kB5[s] - kB4[s];
-        }
-        op.x0 = b->x, op.y0 = b->y;
-        // FSAL derivative for runs that have none yet (src/timestep.cpp:111-112).
-        op.kind = pv::kOpStore;
-        op.ku = b->kx[0], op.kv = b->ky[0];
-        launch(b, b->x, b->y, op, true, pv::kPassNeedK0);
-        // Stage positions: 1 -> A, 2 -> B, 3 -> A, 4 -> B, 5 -> A, 6 (fifth order) -> next.
-        double *px[7] = {nullptr, b->xa, b->xb, b->xa, b->xb, b->xa, b->xn};
-        double *py[7] = {nullptr, b->ya, b->yb, b->ya, b->yb, b->ya, b->yn};
-        op.kind = pv::kOpStage, op.cur = 0, op.coef[0] = kDopriA[1][0];
-        op.xn = px[1], op.yn = py[1];
-        pv::launch_batch_stage1(op, b->owner, b->steps_d, n, b->stream);
-        hip_check(hipGetLastError(), "launch HIP ensemble stage");
-        for (int s = 1; s < 7; ++s) {
-            op.cur = s;
-            op.ku = b->kx[s], op.kv = b->ky[s];
-            if (s < 6) {
-                op.kind = pv::kOpStage;
-                for (int j = 0; j <= s; ++j)
-                    op.coef[j] = kDopriA[s + 1][j];
-                op.xn = px[s + 1], op.yn = py[s + 1];
-            } else {
-                op.kind = pv::kOpDopriError;
-                op.xn = nullptr, op.yn = nullptr;
-            }
-            launch(b, px[s], py[s], op, true, pv::kPassAll);
-        }
-        pv::launch_batch_control(b->steps_d, (int)b->max_inst, b->stream);
-        hip_check(hipGetLastError(), "launch HIP ensemble controller");
-        pv::launch_batch_accept(b->owner, b->steps_d, b->x, b->y, b->xn, b->yn, b->kx[0],
-                                b->ky[0], b->kx[6], b->ky[6], n, b->stream);
-        hip_check(hipGetLastError(), "launch HIP ensemble accept");
+        // Runs that finish or fail part-way sit out the rest: every kernel of an attempt skips
+        // a run that is not running.  The work items are rebuilt at the next call.
+        for (int c = 0; c < count; ++c)
+            enqueue_attempt(b);
         hip_check(hipMemcpyAsync(b->steps_h, b->steps_d, b->max_inst * sizeof(pv::InstanceStep),
                                  hipMemcpyDeviceToHost, b->stream),
                   "read instance states");
@@ -880,6 +890,8 @@ pv_status pv_batch_attempt(pv_batch *b) {
                 b->items_dirty = true;
     });
 }
+
+pv_status pv_batch_attempt(pv_batch *b) { return pv_batch_attempts(b, 1); }
 
 pv_status pv_batch_poll(pv_batch *b, pv_slot_state *states, size_t cap, size_t *count) {
     return guard([&] {

@@ -156,8 +156,12 @@ class Ensemble:
                                        ctypes.byref(slot)))
         return None if slot.value < 0 else slot.value
 
-    def attempt(self):
-        _check(self.lib.pv_batch_attempt(self._h))
+    def attempt(self, count: int = 1):
+        """`count` DOPRI5 attempts of every running run, one synchronisation at the end."""
+        if count == 1:
+            _check(self.lib.pv_batch_attempt(self._h))
+        else:
+            _check(self.lib.pv_batch_attempts(self._h, int(count)))
 
     def poll(self) -> List[SlotState]:
         cap = self.max_instances

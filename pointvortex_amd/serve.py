@@ -6,7 +6,11 @@ budget -- 208 bytes per vortex of ensemble state, plus the partial sums of the s
 decomposition (16 bytes per vortex per 128 vortices of the run) -- so new runs join runs
 already in flight; finished or failed runs leave and free their bytes.  Each run keeps its
 state, FSAL derivative and step-size controller on the device; one attempt advances every
-running run by one DOPRI5 attempt (seven stage evaluations).  Runs are split into pairs of
+running run by one DOPRI5 attempt (seven stage evaluations).  Between polls the loop runs up
+to `max_batch` attempts back to back with one synchronisation: one while an arrived request
+waits for budget, never past the next arrival, and never more than a quarter of the attempts
+the youngest running run has taken, so that a run finishes at most about a quarter of its
+attempts before the loop sees it.  Runs are split into pairs of
 128-vortex tiles, each unordered pair computed once (`Ensemble.set_symmetric`), so a lone
 large run still spreads over the GPU.  Latency is measured from arrival to completion.
 
@@ -90,7 +94,7 @@ class ServerStats:
 class Server:
     def __init__(self, hbm_budget_bytes: int, max_runs: int = 4096, device: int = 0,
                  mixed_rtol_threshold: float = 1e-6, keep_results: bool = False,
-                 symmetric: bool = True):
+                 symmetric: bool = True, max_batch: int = 16):
         self.per_vortex = bytes_per_vortex()
         self.budget = int(hbm_budget_bytes)
         self.capacity = max(1, self.budget // self.per_vortex)
@@ -101,6 +105,7 @@ class Server:
         self.threshold = mixed_rtol_threshold
         self.keep = keep_results
         self.symmetric = symmetric
+        self.max_batch = max(1, int(max_batch))
         self.engines: Dict[str, Ensemble] = {}
 
     def footprint(self, n: int) -> int:
@@ -124,6 +129,8 @@ class Server:
         leaves."""
         pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
         active: Dict[tuple, tuple] = {}
+        tried: Dict[tuple, int] = {}       # attempts each running run has taken so far
+        per_attempt = 0.0                  # running average of the wall time of one attempt
         stats = ServerStats()
         t0 = clock()
         while pending or active:
@@ -153,16 +160,29 @@ class Server:
                 if wait > 0:
                     time.sleep(min(wait, 0.01))
                 continue
+            batch = 1
+            if self.max_batch > 1 and not (pending and pending[0].arrival <= now):
+                batch = min(self.max_batch, min(tried.get(k, 0) for k in active) // 4)
+                if pending and per_attempt > 0.0:
+                    batch = min(batch, int((pending[0].arrival - now) / per_attempt))
+                batch = max(1, batch)
+            started = clock()
             for prec, eng in self.engines.items():
                 if any(k[0] == prec for k in active):
-                    eng.attempt()
-                    stats.attempts += 1
+                    eng.attempt(batch)
+                    stats.attempts += batch
+            elapsed = (clock() - started) / batch
+            per_attempt = elapsed if per_attempt == 0.0 else 0.8 * per_attempt + 0.2 * elapsed
             now = clock() - t0
             for prec, eng in self.engines.items():
                 for s in eng.poll():
                     key = (prec, s.slot)
-                    if key not in active or s.state == "running":
+                    if key not in active:
                         continue
+                    if s.state == "running":
+                        tried[key] = s.accepted + s.rejected
+                        continue
+                    tried.pop(key, None)
                     req, admitted = active.pop(key)
                     x = y = None
                     if self.keep and s.state == "finished":

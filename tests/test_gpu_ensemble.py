@@ -163,6 +163,37 @@ def test_adaptive_run_alone_equals_run_in_the_mix(precision, mode):
     assert all(s.state == "finished" for s in everyone)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_attempts_match_single_attempts(precision):
+    """pv_batch_attempts(b, k) against k single attempts: the same states and positions bit for
+    bit, including runs that finish part-way through a batch (they sit out the rest)."""
+    cfgs, pops = mixed_population(5, 21, n_min=20, n_max=200)
+    ends = [0.002, 0.0003, 0.002, 0.0005, 0.002]
+
+    def make():
+        e = Ensemble(4096, 8, precision)
+        e.set_symmetric(1)
+        return e, [e.admit(c, p.x, p.y, p.circulation, t_end=t, dt0=1e-4, atol=1e-10, rtol=1e-8)
+                   for c, p, t in zip(cfgs, pops, ends)]
+
+    one, s1 = make()
+    many, s8 = make()
+    for _ in range(24):
+        one.attempt()
+    for _ in range(3):
+        many.attempt(8)
+    p1 = {s.slot: s for s in one.poll()}
+    p8 = {s.slot: s for s in many.poll()}
+    for a, b, p in zip(s1, s8, pops):
+        assert (p1[a].state, p1[a].t, p1[a].dt, p1[a].accepted, p1[a].rejected) == \
+            (p8[b].state, p8[b].t, p8[b].dt, p8[b].accepted, p8[b].rejected)
+        xa, ya = one.fetch(a, p.n)
+        xb, yb = many.fetch(b, p.n)
+        assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
+    assert sum(p1[a].state == "finished" for a in s1) >= 1
+    assert sum(p1[a].state == "running" for a in s1) >= 1
+
+
 def test_adaptive_matches_single_handle_dopri5():
     """One adaptive run against the single-handle DOPRI5 path driven to the same end time with
     the reference's landing rule (src/main.cpp:312-314)."""
