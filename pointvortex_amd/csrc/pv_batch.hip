@@ -187,7 +187,7 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
     double xi[TPT], yi[TPT];
     typename P::SymScalar gi[TPT];
     int ti[TPT];
-    bool near = false;
+    bool near = false, sym_near = false;
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
         const int i = t0 + t * 64 + lane;
@@ -199,6 +199,8 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
         ti[t] = valid ? i : -1;
         if constexpr (P::kNearTiles)
             near |= valid && P::is_near(gp, xi[t], yi[t]);
+        if constexpr (P::kSymNear)
+            sym_near |= valid && P::is_sym_near(gp, xi[t], yi[t]);
     }
     wave_sync();
     for (int k = lane; k < span; k += 64) {
@@ -207,10 +209,14 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
         sl->src[k] = P::src(gp, xj, yj, k < cnt ? a.g[j] : 0.0);
         if constexpr (P::kNearTiles)
             near |= k < cnt && P::is_near(gp, xj, yj);
+        if constexpr (P::kSymNear)
+            sym_near |= k < cnt && P::is_sym_near(gp, xj, yj);
     }
     wave_sync();
     if constexpr (P::kNearTiles)
         near = __any(near);
+    if constexpr (P::kSymNear)
+        sym_near = __any(sym_near);
 
     Acc acc[TPT];
 #pragma unroll
@@ -283,30 +289,38 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
             for (int t = 0; t < TPT; ++t)
                 tg[t] = P::sym_tgt(gp, xi[t], yi[t]);
             // The ring of the single-system symmetric kernel: 64 steps per group of 64
-            // sources bring every source's reaction back to its first lane.
-            for (int g0 = 0; g0 < span; g0 += 64) {
-                typename P::SymR r;
-                r.u = 0;
-                r.v = 0;
-                typename P::Src next = sl->src[g0 + lane];
+            // sources bring every source's reaction back to its first lane.  `direct` is the
+            // near-centre form of kSymNear policies (wave-uniform).
+            const auto ring = [&](auto direct) {
+                constexpr bool kDirect = decltype(direct)::value;
+                for (int g0 = 0; g0 < span; g0 += 64) {
+                    typename P::SymR r;
+                    r.u = 0;
+                    r.v = 0;
+                    typename P::Src next = sl->src[g0 + lane];
 #pragma unroll 2
-                for (int s = 0; s < 64; ++s) {
-                    const typename P::Src s_a = next;
-                    next = sl->src[g0 + ((lane + dir * (s + 1)) & 63)];
-                    sym_step<P, TPT>(gp, tg, gi, s_a, acc, r);
-                    r.u = wave_ror1(r.u);
-                    r.v = wave_ror1(r.v);
-                }
-                if (g0 + lane < cnt) {
-                    part[(size_t)(2 * it.I) * nn + s0 + g0 + lane] = (double)r.u;
-                    part[(size_t)(2 * it.I + 1) * nn + s0 + g0 + lane] = (double)r.v;
-                }
-                if constexpr (P::kFoldEvery > 0) {
+                    for (int s = 0; s < 64; ++s) {
+                        const typename P::Src s_a = next;
+                        next = sl->src[g0 + ((lane + dir * (s + 1)) & 63)];
+                        sym_step<P, TPT, kDirect>(gp, tg, gi, s_a, acc, r);
+                        r.u = wave_ror1(r.u);
+                        r.v = wave_ror1(r.v);
+                    }
+                    if (g0 + lane < cnt) {
+                        part[(size_t)(2 * it.I) * nn + s0 + g0 + lane] = (double)r.u;
+                        part[(size_t)(2 * it.I + 1) * nn + s0 + g0 + lane] = (double)r.v;
+                    }
+                    if constexpr (P::kFoldEvery > 0) {
 #pragma unroll
-                    for (int t = 0; t < TPT; ++t)
-                        P::fold(acc[t]);
+                        for (int t = 0; t < TPT; ++t)
+                            P::fold(acc[t]);
+                    }
                 }
-            }
+            };
+            if (P::kSymNear && sym_near)
+                ring(Flag<P::kSymNear>{});
+            else
+                ring(Flag<false>{});
         }
     }
 #pragma unroll

@@ -252,6 +252,7 @@ struct InfF64 {
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = true;
     static constexpr bool kNearTiles = false;
+    static constexpr bool kSymNear = false;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, kInvTwoPi * g, 0.0};
     }
@@ -334,6 +335,7 @@ struct InfMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
+    static constexpr bool kSymNear = false;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
     }
@@ -423,6 +425,7 @@ struct DiskF64 {
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = true;
     static constexpr bool kNearTiles = true;
+    static constexpr bool kSymNear = false;
     __device__ static bool is_near(const GeoParams &p, double x, double y) {
         return __builtin_fma(x, x, y * y) < 0x1p-200 * p.R2;
     }
@@ -564,7 +567,7 @@ struct DiskF64 {
 struct DiskMixed {
     struct Src {
         double x, y;
-        float g, ix, iy, pad;      // g = G/(2 pi); (ix, iy) = delta
+        float g, ix, iy, w;        // g = G/(2 pi); (ix, iy) = delta; w = 1/|x|^2 (0 near the centre)
     };
     struct Tgt {
         double x, y;
@@ -576,6 +579,7 @@ struct DiskMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
+    static constexpr bool kSymNear = true;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         const double r2 = __builtin_fma(x, x, y * y);
         float ix = 1e30f, iy = 0.f;
@@ -584,7 +588,8 @@ struct DiskMixed {
             ix = (float)fmin(fmax(x * q, -1e30), 1e30);
             iy = (float)fmin(fmax(y * q, -1e30), 1e30);
         }
-        return {x, y, (float)(kInvTwoPi * g), ix, iy, 0.f};
+        const float w = r2 >= 0x1p-80 * p.R2 ? (float)(1.0 / r2) : 0.f;
+        return {x, y, (float)(kInvTwoPi * g), ix, iy, w};
     }
     __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
     __device__ static void zero(Acc &a) {
@@ -626,27 +631,35 @@ struct DiskMixed {
         v = a.v.value();
     }
     // Symmetric evaluation: d = x_i - x_j once (FP64 difference, FP32 after).  j's image at i is
-    // d - delta_j, i's image at j is x_j - x_i* = -d - delta_i (the target carries delta_i); each
-    // side keeps its own FP32 reciprocal.  The reaction on the source is an FP32 sum (over at
-    // most 64 x 4 targets) before it joins the FP64 partials.
+    // d - delta_j, i's image at j is x_j - x_i* = -d - delta_i (the target carries delta_i).  The
+    // two image terms share one FP32 reciprocal, as in DiskF64: 1/|x_j - x_i*|^2 =
+    // |x_i|^2 w_j / |x_i - x_j*|^2.  That needs w_j = 1/|x_j|^2 and the product in FP32 range,
+    // so blocks holding a vortex within 2^-40 R of the centre (is_sym_near) take the direct form
+    // (kDirect), with its own reciprocal for i's image.  The reaction on the source is an FP32
+    // sum (over at most 64 x 4 targets) before it joins the FP64 partials.
     static constexpr bool kSym = true;
     static constexpr int kSymDenoms = 0;
     struct SymTgt {
         double x, y;
         float ix, iy;           // delta_i
+        float r2;               // |x_i|^2
     };
     struct SymTmp {};
     using SymScalar = float;
     struct SymR {
         float u, v;
     };
+    __device__ static bool is_sym_near(const GeoParams &p, double x, double y) {
+        return __builtin_fma(x, x, y * y) < 0x1p-80 * p.R2;
+    }
     __device__ static float sym_g(const GeoParams &, double g) { return (float)(kInvTwoPi * g); }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) {
         const Src s = src(p, x, y, 0.0);
-        return {x, y, s.ix, s.iy};
+        return {x, y, s.ix, s.iy, (float)__builtin_fma(x, x, y * y)};
     }
-    __device__ static void sym_pair(const GeoParams &, const SymTgt &t, float gi, const Src &s,
-                                    Acc &a, SymR &r) {
+    template <bool kDirect>
+    __device__ static void sym_pair_t(const GeoParams &, const SymTgt &t, float gi, const Src &s,
+                                      Acc &a, SymR &r) {
         const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
         const float inv = rcp32(__builtin_fmaf(dx, dx, dy * dy));
         const float c = s.g * inv;
@@ -655,11 +668,19 @@ struct DiskMixed {
         const float ci = gi * inv;
         r.u = __builtin_fmaf(ci, dy, r.u);
         r.v = __builtin_fmaf(-ci, dx, r.v);
-        image(s, dx, dy, a);                       // -G_j at x_j*, on i
+        const float ex = dx - s.ix, ey = dy - s.iy;
+        const float inv_e = rcp32(__builtin_fmaf(ex, ex, ey * ey));
+        const float k = s.g * inv_e;               // -G_j at x_j*, on i
+        a.u32 = __builtin_fmaf(k, ey, a.u32);
+        a.v32 = __builtin_fmaf(-k, ex, a.v32);
         const float fx = -dx - t.ix, fy = -dy - t.iy;
-        const float k = gi * rcp32(__builtin_fmaf(fx, fx, fy * fy));
-        r.u = __builtin_fmaf(k, fy, r.u);          // -G_i at x_i*, on j
-        r.v = __builtin_fmaf(-k, fx, r.v);
+        float kf;                                  // -G_i at x_i*, on j
+        if constexpr (kDirect)
+            kf = gi * rcp32(__builtin_fmaf(fx, fx, fy * fy));
+        else
+            kf = (gi * t.r2) * (s.w * inv_e);      // G_i |x_i|^2 is the same for every source
+        r.u = __builtin_fmaf(kf, fy, r.u);
+        r.v = __builtin_fmaf(-kf, fx, r.v);
     }
 };
 
@@ -686,6 +707,7 @@ struct PerF64 {
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
+    static constexpr bool kSymNear = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, p.scale * g, 0.0};
     }
@@ -795,6 +817,7 @@ struct PerMixed {
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
+    static constexpr bool kSymNear = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
     }

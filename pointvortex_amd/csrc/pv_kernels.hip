@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
     double xi[TPT], yi[TPT];
     typename P::SymScalar gi[TPT];
     int ti[TPT];
-    int near = 0;
+    int near = 0, sym_near = 0;
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
         const int i = t0 + t * kBlock + tid;
@@ -240,11 +240,18 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
         ti[t] = valid ? i : -1;
         if constexpr (P::kNearTiles)
             near |= valid && P::is_near(gp, xi[t], yi[t]);
+        if constexpr (P::kSymNear)
+            sym_near |= valid && P::is_sym_near(gp, xi[t], yi[t]);
     }
     if constexpr (P::kNearTiles) {
         for (int k = tid; k < s1 - s0; k += kBlock)
             near |= P::is_near(gp, a.x[s0 + k], a.y[s0 + k]);
         near = __syncthreads_or(near);
+    }
+    if constexpr (P::kSymNear) {
+        for (int k = tid; k < s1 - s0; k += kBlock)
+            sym_near |= P::is_sym_near(gp, a.x[s0 + k], a.y[s0 + k]);
+        sym_near = __syncthreads_or(sym_near);
     }
 
     Acc acc[TPT];
@@ -328,12 +335,14 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
                     done = true;
                 }
             }
-            if (!done) {
-                // Ring: lane l starts on source g0 + l with that source's reaction in r; after
-                // each step r moves one lane (wave_ror:1) and the lane reads the source whose
-                // reaction it now holds.  64 steps bring every reaction back to its first lane
-                // with the sum over the wave's targets.  Staged pads (strength 0) fill the last
-                // group of 64; their reactions are dropped.
+            // Ring: lane l starts on source g0 + l with that source's reaction in r; after each
+            // step r moves one lane (wave_ror:1) and the lane reads the source whose reaction it
+            // now holds.  64 steps bring every reaction back to its first lane with the sum
+            // over the wave's targets.  Staged pads (strength 0) fill the last group of 64;
+            // their reactions are dropped.  `direct` is the near-centre form of kSymNear
+            // policies (block-uniform, so each form is its own loop).
+            const auto ring = [&](auto direct) {
+                constexpr bool kDirect = decltype(direct)::value;
                 for (int g0 = 0; g0 < cnt; g0 += 64) {
                     typename P::SymR r;
                     r.u = 0;
@@ -345,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
                         // the lane's first source again, unused).
                         const typename P::Src s_a = next;
                         next = sh[g0 + ((lane + dir * (s + 1)) & 63)];
-                        sym_step<P, TPT>(gp, tg, gi, s_a, acc, r);
+                        sym_step<P, TPT, kDirect>(gp, tg, gi, s_a, acc, r);
                         r.u = wave_ror1(r.u);
                         r.v = wave_ror1(r.v);
                     }
@@ -354,6 +363,12 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
                         red[wave][g0 + lane][1] = (double)r.v;
                     }
                 }
+            };
+            if (!done) {
+                if (P::kSymNear && sym_near)
+                    ring(Flag<P::kSymNear>{});
+                else
+                    ring(Flag<false>{});
             }
             if constexpr (P::kFoldEvery > 0) {
 #pragma unroll

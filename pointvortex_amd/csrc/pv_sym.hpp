@@ -29,6 +29,11 @@ __device__ __forceinline__ float wave_ror1(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppWaveRor1, 0xF, 0xF, true));
 }
 
+// A compile-time flag passed as a value (to instantiate a generic lambda both ways).
+template <bool B> struct Flag {
+    static constexpr bool value = B;
+};
+
 // The ring's step in lane units (1 or 63, i.e. -1 mod 64): after k rotations lane l holds what
 // lane (l + k dir) mod 64 started with.  Read from the rotation itself, so the ring does not
 // depend on which way the hardware turns; the same for every lane.
@@ -39,7 +44,8 @@ __device__ __forceinline__ int ring_dir(int lane) {
 
 // One source against a lane's targets: the target side into acc[t], the reaction on the source
 // into r.  The denominators of all TPT pairs share one reciprocal through the product tree.
-template <class P, int TPT>
+// kDirect: the block holds a vortex near the centre (P::kSymNear policies; P::is_sym_near).
+template <class P, int TPT, bool kDirect = false>
 __device__ __forceinline__ void sym_step(const GeoParams &gp, const typename P::SymTgt *tg,
                                          const typename P::SymScalar *gi,
                                          const typename P::Src &s, typename P::Acc *acc,
@@ -62,8 +68,12 @@ __device__ __forceinline__ void sym_step(const GeoParams &gp, const typename P::
             P::sym_apply(s, gi[t], m[t], &inv[t * ND], acc[t], r);
     } else {
 #pragma unroll
-        for (int t = 0; t < TPT; ++t)
-            P::sym_pair(gp, tg[t], gi[t], s, acc[t], r);
+        for (int t = 0; t < TPT; ++t) {
+            if constexpr (P::kSymNear)
+                P::template sym_pair_t<kDirect>(gp, tg[t], gi[t], s, acc[t], r);
+            else
+                P::sym_pair(gp, tg[t], gi[t], s, acc[t], r);
+        }
     }
 }
 

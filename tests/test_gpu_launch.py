@@ -140,11 +140,14 @@ def test_symmetric_mixed_keeps_parity(oracle, geometry, n, tpt, tile):
     assert err <= max(3.0 * err_split, 1e-7), (err, err_split)
 
 
-@pytest.mark.parametrize("pos", [(2e-160, 1e-161), (-3e-31, 2e-31)])
+@pytest.mark.parametrize("pos", [(2e-160, 1e-161), (-3e-31, 2e-31), (6e-13, -4e-13),
+                                 (8e-13, 5e-13), (3e-7, 1e-7)])
 def test_symmetric_mixed_near_centre_disk_source(oracle, pos):
     """A near-centre vortex (its image a far, finite offset) in a middle tile, so that it is a
-    source for earlier tiles and a target for later ones.  One such vortex only: two of them
-    would sit closer than FP32 can square."""
+    source for earlier tiles and a target for later ones.  Below 2^-40 R (about 9.1e-13) its
+    blocks take the direct form of i's image term; above it, the shared reciprocal with
+    w = 1/|x|^2 up to about 1e24.  One such vortex only: two of them would sit closer than
+    FP32 can square."""
     pop, k = setup("disk", "mixed", 1800, seed=73)
     x, y = pop.x.copy(), pop.y.copy()
     x[900], y[900] = pos
@@ -153,6 +156,8 @@ def test_symmetric_mixed_near_centre_disk_source(oracle, pos):
     assert np.all(np.isfinite(vel.x)) and np.all(np.isfinite(vel.y))
     ur, vr = oracle.evaluate(spec("disk"), x, y, pop.circulation)
     assert normwise(vel.x, vel.y, ur, vr) <= 1e-6
+    near = np.hypot(vel.x[900] - ur[900], vel.y[900] - vr[900]) / np.hypot(ur[900], vr[900])
+    assert near <= 1e-5
 
 
 def test_symmetric_path_near_centre_disk_source(oracle):
