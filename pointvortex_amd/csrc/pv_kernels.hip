@@ -302,8 +302,8 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
             const int span = (cnt + 63) & ~63;          // <= TILE: TILE is a multiple of 64
             __syncthreads();
             for (int k = tid; k < span; k += kBlock)
-                sh[k] = k < cnt ? P::src(gp, a.x[base + k], a.y[base + k], a.g[base + k])
-                                : P::src(gp, a.x[base], a.y[base], 0.0);
+                sh[k] = k < cnt ? ring_src<P>(gp, a.x[base + k], a.y[base + k], a.g[base + k])
+                                : ring_src<P>(gp, a.x[base], a.y[base], 0.0);
             __syncthreads();
             bool done = false;
             if constexpr (P::kNearTiles) {

@@ -42,6 +42,16 @@ __device__ __forceinline__ int ring_dir(int lane) {
     return __builtin_amdgcn_readfirstlane((from - lane) & 63);
 }
 
+// A source record for the symmetric ring: the policy's sym_src where it has one (kSymNear).
+template <class P>
+__device__ __forceinline__ typename P::Src ring_src(const GeoParams &gp, double x, double y,
+                                                    double g) {
+    if constexpr (P::kSymNear)
+        return P::sym_src(gp, x, y, g);
+    else
+        return P::src(gp, x, y, g);
+}
+
 // One source against a lane's targets: the target side into acc[t], the reaction on the source
 // into r.  The denominators of all TPT pairs share one reciprocal through the product tree.
 // kDirect: the block holds a vortex near the centre (P::kSymNear policies; P::is_sym_near).
