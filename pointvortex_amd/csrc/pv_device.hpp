@@ -297,7 +297,6 @@ struct InfF64 {
     }
     // Symmetric evaluation (each unordered pair once): the term on j from i reuses the
     // difference and the inverse with i's circulation and the opposite sign of (dx, dy).
-    static constexpr bool kSym = true;
     static constexpr int kSymDenoms = 1;
     using SymTgt = Tgt;
     using SymTmp = Tmp;
@@ -373,7 +372,6 @@ struct InfMixed {
     // Symmetric evaluation: one FP64 difference, one FP32 reciprocal per unordered pair; the
     // reaction on the source is an FP32 sum (over at most 64 x 4 targets) like the target
     // side's per-tile FP32 sums, and is added to the FP64 partials after that.
-    static constexpr bool kSym = true;
     static constexpr int kSymDenoms = 0;
     using SymTgt = Tgt;
     struct SymTmp {};
@@ -510,7 +508,6 @@ struct DiskF64 {
     // symmetric in i and j, so 1/|x_j - x_i*|^2 = |x_i|^2 w_j / |x_i - x_j*|^2.  Both
     // differences are still formed directly (no cancellation); only the reciprocal is shared.
     // Blocks holding a near-centre vortex use the reference's form both ways.
-    static constexpr bool kSym = true;
     static constexpr int kSymDenoms = 2;
     using SymScalar = double;
     using SymR = Acc;
@@ -636,7 +633,6 @@ struct DiskMixed {
     // so blocks holding a vortex within 2^-40 R of the centre (is_sym_near) take the direct form
     // (kDirect), with its own reciprocal for i's image.  The reaction on the source is an FP32
     // sum (over at most 64 x 4 targets) before it joins the FP64 partials.
-    static constexpr bool kSym = true;
     static constexpr int kSymDenoms = 0;
     struct SymTgt {
         double x, y;
@@ -785,7 +781,6 @@ struct PerF64 {
     }
     // Symmetric evaluation: with X, Y -> -X, -Y every factor is odd or even, so the term on j
     // from i is the same (fu, fv) with i's circulation and the opposite sign.
-    static constexpr bool kSym = true;
     static constexpr int kSymDenoms = 0;
     using SymTgt = Tgt;
     struct SymTmp {};
@@ -892,7 +887,6 @@ struct PerMixed {
     }
     // Symmetric evaluation as in PerF64 (the term is odd under X, Y -> -X, -Y); the reaction on
     // the source is an FP32 sum (over at most 64 x 4 targets) before it joins the FP64 partials.
-    static constexpr bool kSym = true;
     static constexpr int kSymDenoms = 0;
     using SymTgt = Tgt;
     struct SymTmp {};
