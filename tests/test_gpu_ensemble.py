@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle.pyoracle import spec
-from pointvortex_amd import HipKernel, SimParams, VortexSystem
+from pointvortex_amd import HipKernel, InvalidArgument, SimParams, VortexSystem
 from pointvortex_amd import workloads as wl
 from pointvortex_amd.ensemble import Ensemble, InstanceConfig, bytes_per_vortex
 from tests.conftest import fp64_gate, normwise
@@ -192,6 +192,8 @@ def test_batched_attempts_match_single_attempts(precision):
         assert np.array_equal(xa, xb) and np.array_equal(ya, yb)
     assert sum(p1[a].state == "finished" for a in s1) >= 1
     assert sum(p1[a].state == "running" for a in s1) >= 1
+    with pytest.raises(InvalidArgument, match="attempt count must be positive"):
+        many.attempt(0)
 
 
 def test_adaptive_matches_single_handle_dopri5():
