@@ -672,21 +672,29 @@ struct DiskMixed {
         a.u32 = __builtin_fmaf(-c, dy, a.u32);
         a.v32 = __builtin_fmaf(c, dx, a.v32);
         const float ci = gi * inv;
-        r.u = __builtin_fmaf(ci, dy, r.u);
-        r.v = __builtin_fmaf(-ci, dx, r.v);
+        if constexpr (kDirect) {
+            r.u = __builtin_fmaf(ci, dy, r.u);
+            r.v = __builtin_fmaf(-ci, dx, r.v);
+        }
         const float ex = dx - s.ix, ey = dy - s.iy;
         const float inv_e = rcp32(__builtin_fmaf(ex, ex, ey * ey));
         const float k = s.g * inv_e;               // -G_j at x_j*, on i
         a.u32 = __builtin_fmaf(k, ey, a.u32);
         a.v32 = __builtin_fmaf(-k, ex, a.v32);
-        const float fx = -dx - t.ix, fy = -dy - t.iy;
-        float kf;                                  // -G_i at x_i*, on j
-        if constexpr (kDirect)
-            kf = gi * rcp32(__builtin_fmaf(fx, fx, fy * fy));
-        else
-            kf = (gi * t.r2) * (s.w * inv_e);      // G_i |x_i|^2 is the same for every source
-        r.u = __builtin_fmaf(kf, fy, r.u);
-        r.v = __builtin_fmaf(-kf, fx, r.v);
+        if constexpr (kDirect) {
+            const float fx = -dx - t.ix, fy = -dy - t.iy;
+            const float kf = gi * rcp32(__builtin_fmaf(fx, fx, fy * fy));   // -G_i at x_i*
+            r.u = __builtin_fmaf(kf, fy, r.u);
+            r.v = __builtin_fmaf(-kf, fx, r.v);
+        } else {
+            // -G_i at x_i*, on j; G_i |x_i|^2 is the same for every source.  With
+            // f = -d - delta_i the direct and image reactions combine:
+            // ci (dy, -dx) + kf (fy, -fx) = (ci - kf) (dy, -dx) + kf (-delta_iy, delta_ix).
+            const float kf = (gi * t.r2) * (s.w * inv_e);
+            const float m = ci - kf;
+            r.u = __builtin_fmaf(-kf, t.iy, __builtin_fmaf(m, dy, r.u));
+            r.v = __builtin_fmaf(kf, t.ix, __builtin_fmaf(-m, dx, r.v));
+        }
     }
 };
 
