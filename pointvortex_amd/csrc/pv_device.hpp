@@ -512,15 +512,15 @@ struct DiskF64 {
     using SymScalar = double;
     using SymR = Acc;
     struct SymTgt {
-        double x, y, ix, iy, r2;
+        double x, y, jx, jy, r2;               // jx, jy: delta_i = x_i* - x_i
     };
     struct SymTmp {
-        double dx, dy, ex, ey, fx, fy, r2, w;  // x_i - x_j, x_i - x_j*, x_j - x_i*, |x_i|^2, w_j
+        double dx, dy, ex, ey, r2, w, jx, jy;  // x_i - x_j, x_i - x_j*, |x_i|^2, w_j, delta_i
     };
     __device__ static double sym_g(const GeoParams &, double g) { return kInvTwoPi * g; }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) {
         const Src s = src(p, x, y, 0.0);
-        return {x, y, s.ix, s.iy, __builtin_fma(x, x, y * y)};
+        return {x, y, s.ix - x, s.iy - y, __builtin_fma(x, x, y * y)};
     }
     __device__ static void sym_prep(const GeoParams &, const SymTgt &t, const Src &s, SymTmp &m,
                                     double *d) {
@@ -530,10 +530,10 @@ struct DiskF64 {
         m.ex = t.x - s.ix;
         m.ey = t.y - s.iy;
         d[1] = __builtin_fma(m.ex, m.ex, m.ey * m.ey);
-        m.fx = s.x - t.ix;
-        m.fy = s.y - t.iy;
         m.r2 = t.r2;
         m.w = s.w;
+        m.jx = t.jx;
+        m.jy = t.jy;
     }
     __device__ static void sym_apply(const Src &s, double gi, const SymTmp &m, const double *inv,
                                      Acc &a, Acc &r) {
@@ -543,13 +543,14 @@ struct DiskF64 {
         const double k = s.g * inv[1];             // j's image has strength -G_j
         a.u = __builtin_fma(k, m.ey, a.u);
         a.v = __builtin_fma(-k, m.ex, a.v);
+        // The reaction: G_i on j and i's image (strength -G_i; G_i |x_i|^2 is the same for
+        // every source of the loop) on j.  With x_j - x_i* = -d - delta_i the two combine:
+        // ci (dy, -dx) + ki (fy, -fx) = (ci - ki) (dy, -dx) + ki (-delta_iy, delta_ix).
         const double ci = gi * inv[0];
-        r.u = __builtin_fma(ci, m.dy, r.u);
-        r.v = __builtin_fma(-ci, m.dx, r.v);
-        // i's image has strength -G_i; G_i |x_i|^2 is the same for every source of the loop.
         const double ki = (gi * m.r2) * (inv[1] * m.w);
-        r.u = __builtin_fma(ki, m.fy, r.u);
-        r.v = __builtin_fma(-ki, m.fx, r.v);
+        const double cm = ci - ki;
+        r.u = __builtin_fma(-ki, m.jy, __builtin_fma(cm, m.dy, r.u));
+        r.v = __builtin_fma(ki, m.jx, __builtin_fma(-cm, m.dx, r.v));
     }
 };
 
