@@ -24,8 +24,9 @@ namespace pv {
 constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr double kInvTwoPi = 0.5 / kPi;  // src/compute.cpp:97 (inverseTwoPi)
 constexpr int kMaxTerms = 16;            // capacity of the Fourier weight tables
-constexpr int kTermsF64 = 12;            // periodic Fourier terms, FP64 (tail < 2e-17)
-constexpr int kTermsF32 = 6;             // periodic Fourier terms, mixed (tail < 1e-8)
+constexpr int kTermsF64 = 12;            // Hamiltonian Fourier terms, rows |m| >= 1
+constexpr int kImageTermsF64 = 4;        // velocity Fourier terms, rows |m| >= 2, FP64
+constexpr int kImageTermsF32 = 2;        // velocity Fourier terms, rows |m| >= 2, mixed
 
 // Geometry constants, filled on the host (pv_api.cpp) from pv_config.
 struct GeoParams {
@@ -36,10 +37,13 @@ struct GeoParams {
     double wave;         // 2 pi / L                (src/compute.cpp:213)
     double scale;        // 0.5 / L                 (src/compute.cpp:214)
     double R, invR, R2;  // disk radius, 1/R, R^2
-    double G4[kMaxTerms + 1];   // 4 * sum_{m=1..L} exp(-2 pi n m), n = 1..kMaxTerms
-    double H4[kMaxTerms + 1];   // G4[n] / n (Hamiltonian series)
+    double G4[kMaxTerms + 1];   // 4 * sum_{m=2..L} exp(-2 pi n m), n = 1..kMaxTerms (velocity)
+    double H4[kMaxTerms + 1];   // 4 * sum_{m=1..L} exp(-2 pi n m) / n (Hamiltonian series)
     double CL;                  // 2 * sum_{m=1..L} log1p(exp(-4 pi m)) (Hamiltonian)
+    double W1;                  // 4 when the rows m = +-1 exist (L >= 1), else 0
+    double C1, C2, S4;          // cosh 2 pi, 2 cosh 2 pi, 4 sinh^2 2 pi
     float G4f[kMaxTerms + 1];
+    float W1f, C1f, C2f, S4f;
 };
 
 // ---------------------------------------------------------------- device math helpers
@@ -704,11 +708,20 @@ struct DiskMixed {
 // src/compute.cpp:223-249.  With X = k rem(dx, L), Y = k rem(dy, L) in [-pi, pi]:
 //   u -= scale G [ sin Y / (cosh X - cos Y) + sum_{m != 0, |m| <= L} sin Y / (cosh(X - 2 pi m) - cos Y) ]
 // (and v with X, Y exchanged).  The m = 0 row keeps the reference's cancellation-free
-// denominator 2(sinh^2(X/2) + sin^2(Y/2)).  For |X| < 2 pi the image rows have the exact
-// expansion  sum_{m != 0} ... = 4 sum_{n>=1} G_n sin(nY) cosh(nX),  G_n = sum_{m=1..L} e^{-2 pi n m},
-// evaluated with the Chebyshev recurrences for sin(nY) and cosh(nX).  Terms fall as
-// e^{-pi n}; 12 terms reach 2e-17 (FP64), 6 terms 1e-8 (mixed).  The reference drops image
-// terms with |X - 2 pi m| > 40 (below 5e-18 relative); the series keeps them.
+// denominator d0 = 2(sinh^2(X/2) + sin^2(Y/2)).
+// Rows m = +-1 in closed form: with c = cosh X, k = cos Y, C = cosh 2 pi, S = sinh 2 pi,
+// D+- = cosh(X -+ 2 pi) - k = C c -+ S sinh X - k, so D+ + D- = 2 (C c - k) and
+// D+ D- = (C c - k)^2 - S^2 (c^2 - 1) = c^2 - 2 C k c + k^2 + S^2 (C^2 - S^2 = 1).  In the
+// doubled variables ch2 = 2c, cs2 = 2k:  t = C ch2 - cs2 = D+ + D-  and
+// q = ch2 (ch2 - 2 C cs2) + cs2^2 + 4 S^2 = 4 D+ D-.  q >= 4 (C - cosh pi)^2 while its one
+// negative term is below 9% of 4 S^2, so nothing cancels; the two rows add sin Y * 4 t / q and
+// join the m = 0 row over the common denominator d0 q.
+// Rows |m| >= 2: for |X| < 4 pi the exact expansion
+//   sum_{|m| >= 2} ... = 4 sum_{n>=1} G_n sin(nY) cosh(nX),  G_n = sum_{m=2..L} e^{-2 pi n m},
+// evaluated with the Chebyshev recurrences for sin(nY) and cosh(nX).  Term n is at most
+// 4 n e^{-4 pi n} cosh(n pi) |sin Y| times the rows above, so 4 terms leave 2e-19 of the
+// result (FP64) and 2 terms 4e-11 (mixed).  The reference drops image terms with
+// |X - 2 pi m| > 40 (below 5e-18 relative); the closed form and the series keep them.
 struct PerF64 {
     struct Src {
         double x, y, gs, pad;
@@ -728,9 +741,8 @@ struct PerF64 {
     }
     __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
     __device__ static void zero(Acc &a) { a.u = a.v = 0.0; }
-    __device__ static void terms(const GeoParams &p, const Tgt &t, const Src &s, double &tu,
-                                 double &tv, double &d0u, double &d0v, double &sin_x,
-                                 double &sin_y) {
+    __device__ static void terms(const GeoParams &p, const Tgt &t, const Src &s, double &fu,
+                                 double &fv, bool self) {
         const double X = p.wave * remainder_fast(t.x - s.x, p.L, p.invL);
         const double Y = p.wave * remainder_fast(t.y - s.y, p.L, p.invL);
         const double hx = 0.5 * X, hy = 0.5 * Y;
@@ -741,23 +753,23 @@ struct PerF64 {
         const double csx = cos_f64(hx2), csy = cos_f64(hy2);
         const double snx2 = snx * snx, sny2 = sny * sny;
         const double shx2 = shx * shx, shy2 = shy * shy;
-        sin_x = 2.0 * snx * csx;
-        sin_y = 2.0 * sny * csy;
+        const double sin_x = 2.0 * snx * csx;
+        const double sin_y = 2.0 * sny * csy;
         const double cos2x = __builtin_fma(-4.0, snx2, 2.0);   // 2 cos X
         const double cos2y = __builtin_fma(-4.0, sny2, 2.0);
         const double ch2x = __builtin_fma(4.0, shx2, 2.0);     // 2 cosh X
         const double ch2y = __builtin_fma(4.0, shy2, 2.0);
-        d0u = 2.0 * (shx2 + sny2);
-        d0v = 2.0 * (shy2 + snx2);
-        // sin(nY) cosh(nX) and sin(nX) cosh(nY), n = 1..kTermsF64.
+        const double d0u = self ? 1.0 : 2.0 * (shx2 + sny2);
+        const double d0v = self ? 1.0 : 2.0 * (shy2 + snx2);
+        // Rows |m| >= 2: sin(nY) cosh(nX) and sin(nX) cosh(nY), n = 1..kImageTermsF64.
         double sy0 = 0.0, sy1 = sin_y, sx0 = 0.0, sx1 = sin_x;
         double cx0 = 1.0, cx1 = 0.5 * ch2x, cy0 = 1.0, cy1 = 0.5 * ch2y;
         double au = 0.0, av = 0.0;
 #pragma unroll
-        for (int n = 1; n <= kTermsF64; ++n) {
+        for (int n = 1; n <= kImageTermsF64; ++n) {
             au = __builtin_fma(p.G4[n] * sy1, cx1, au);
             av = __builtin_fma(p.G4[n] * sx1, cy1, av);
-            if (n < kTermsF64) {
+            if (n < kImageTermsF64) {
                 const double sy2 = __builtin_fma(cos2y, sy1, -sy0);
                 const double sx2 = __builtin_fma(cos2x, sx1, -sx0);
                 const double cx2 = __builtin_fma(ch2x, cx1, -cx0);
@@ -766,25 +778,28 @@ struct PerF64 {
                 cx0 = cx1, cx1 = cx2, cy0 = cy1, cy1 = cy2;
             }
         }
-        tu = au;
-        tv = av;
+        // Rows 0 and +-1: sin Y (q + W1 t d0) / (d0 q).
+        const double tu = __builtin_fma(p.C1, ch2x, -cos2y);
+        const double tv = __builtin_fma(p.C1, ch2y, -cos2x);
+        const double qu = __builtin_fma(ch2x, __builtin_fma(-p.C2, cos2y, ch2x),
+                                        __builtin_fma(cos2y, cos2y, p.S4));
+        const double qv = __builtin_fma(ch2y, __builtin_fma(-p.C2, cos2x, ch2y),
+                                        __builtin_fma(cos2x, cos2x, p.S4));
+        const double nu = __builtin_fma(p.W1 * tu, d0u, qu);
+        const double nv = __builtin_fma(p.W1 * tv, d0v, qv);
+        fu = __builtin_fma(sin_y * nu, rcp64(d0u * qu), au);
+        fv = __builtin_fma(sin_x * nv, rcp64(d0v * qv), av);
     }
     __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
-        double tu, tv, d0u, d0v, sx, sy;
-        terms(p, t, s, tu, tv, d0u, d0v, sx, sy);
-        const double fu = __builtin_fma(sy, rcp64(d0u), tu);
-        const double fv = __builtin_fma(sx, rcp64(d0v), tv);
+        double fu, fv;
+        terms(p, t, s, fu, fv, false);
         a.u = __builtin_fma(-s.gs, fu, a.u);
         a.v = __builtin_fma(s.gs, fv, a.v);
     }
     __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, Acc &a,
                                         bool self) {
-        double tu, tv, d0u, d0v, sx, sy;
-        terms(p, t, s, tu, tv, d0u, d0v, sx, sy);
-        d0u = self ? 1.0 : d0u;
-        d0v = self ? 1.0 : d0v;
-        const double fu = __builtin_fma(sy, rcp64(d0u), tu);
-        const double fv = __builtin_fma(sx, rcp64(d0v), tv);
+        double fu, fv;
+        terms(p, t, s, fu, fv, self);
         a.u = __builtin_fma(-s.gs, fu, a.u);
         a.v = __builtin_fma(s.gs, fv, a.v);
     }
@@ -799,10 +814,8 @@ struct PerF64 {
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
     __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, double gi, const Src &s,
                                     Acc &a, Acc &r) {
-        double tu, tv, d0u, d0v, sx, sy;
-        terms(p, t, s, tu, tv, d0u, d0v, sx, sy);
-        const double fu = __builtin_fma(sy, rcp64(d0u), tu);
-        const double fv = __builtin_fma(sx, rcp64(d0v), tv);
+        double fu, fv;
+        terms(p, t, s, fu, fv, false);
         a.u = __builtin_fma(-s.gs, fu, a.u);
         a.v = __builtin_fma(s.gs, fv, a.v);
         r.u = __builtin_fma(gi, fu, r.u);
@@ -858,16 +871,16 @@ struct PerMixed {
         const float cos2y = __builtin_fmaf(-4.f, sny2, 2.f);
         const float ch2x = __builtin_fmaf(4.f, shx2, 2.f);
         const float ch2y = __builtin_fmaf(4.f, shy2, 2.f);
-        float d0u = 2.f * (shx2 + sny2);
-        float d0v = 2.f * (shy2 + snx2);
+        const float d0u = self ? 1.f : 2.f * (shx2 + sny2);
+        const float d0v = self ? 1.f : 2.f * (shy2 + snx2);
         float sy0 = 0.f, sy1 = sin_y, sx0 = 0.f, sx1 = sin_x;
         float cx0 = 1.f, cx1 = 0.5f * ch2x, cy0 = 1.f, cy1 = 0.5f * ch2y;
         float au = 0.f, av = 0.f;
 #pragma unroll
-        for (int n = 1; n <= kTermsF32; ++n) {
+        for (int n = 1; n <= kImageTermsF32; ++n) {
             au = __builtin_fmaf(p.G4f[n] * sy1, cx1, au);
             av = __builtin_fmaf(p.G4f[n] * sx1, cy1, av);
-            if (n < kTermsF32) {
+            if (n < kImageTermsF32) {
                 const float sy2 = __builtin_fmaf(cos2y, sy1, -sy0);
                 const float sx2 = __builtin_fmaf(cos2x, sx1, -sx0);
                 const float cx2 = __builtin_fmaf(ch2x, cx1, -cx0);
@@ -876,10 +889,16 @@ struct PerMixed {
                 cx0 = cx1, cx1 = cx2, cy0 = cy1, cy1 = cy2;
             }
         }
-        d0u = self ? 1.f : d0u;
-        d0v = self ? 1.f : d0v;
-        fu = __builtin_fmaf(sin_y, rcp32(d0u), au);
-        fv = __builtin_fmaf(sin_x, rcp32(d0v), av);
+        const float tu = __builtin_fmaf(p.C1f, ch2x, -cos2y);
+        const float tv = __builtin_fmaf(p.C1f, ch2y, -cos2x);
+        const float qu = __builtin_fmaf(ch2x, __builtin_fmaf(-p.C2f, cos2y, ch2x),
+                                        __builtin_fmaf(cos2y, cos2y, p.S4f));
+        const float qv = __builtin_fmaf(ch2y, __builtin_fmaf(-p.C2f, cos2x, ch2y),
+                                        __builtin_fmaf(cos2x, cos2x, p.S4f));
+        const float nu = __builtin_fmaf(p.W1f * tu, d0u, qu);
+        const float nv = __builtin_fmaf(p.W1f * tv, d0v, qv);
+        fu = __builtin_fmaf(sin_y * nu, rcp32(d0u * qu), au);
+        fv = __builtin_fmaf(sin_x * nv, rcp32(d0v * qv), av);
     }
     __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
         float fu, fv;

@@ -88,12 +88,23 @@ inline pv::GeoParams geo_params(const pv_config &c) {
         const long double pi = 3.141592653589793238462643383279502884L;
         for (int n = 1; n <= pv::kMaxTerms; ++n) {
             long double s = 0.0L;
-            for (int m = c.image_layers; m >= 1; --m)
+            for (int m = c.image_layers; m >= 2; --m)
                 s += expl(-2.0L * pi * n * m);
             p.G4[n] = (double)(4.0L * s);
-            p.H4[n] = (double)(4.0L * s / n);
             p.G4f[n] = (float)(4.0L * s);
+            if (c.image_layers >= 1)
+                s += expl(-2.0L * pi * n);
+            p.H4[n] = (double)(4.0L * s / n);
         }
+        const long double ch = coshl(2.0L * pi), sh = sinhl(2.0L * pi);
+        p.W1 = c.image_layers >= 1 ? 4.0 : 0.0;
+        p.C1 = (double)ch;
+        p.C2 = (double)(2.0L * ch);
+        p.S4 = (double)(4.0L * sh * sh);
+        p.W1f = (float)p.W1;
+        p.C1f = (float)ch;
+        p.C2f = (float)(2.0L * ch);
+        p.S4f = (float)(4.0L * sh * sh);
         long double cl = 0.0L;
         for (int m = c.image_layers; m >= 1; --m)
             cl += log1pl(expl(-4.0L * pi * m));
