@@ -35,12 +35,13 @@ struct GeoParams {
     double eps2;         // coreRadius^2 (infinite)
     double L, invL;      // box length and its reciprocal (periodic)
     double wave;         // 2 pi / L                (src/compute.cpp:213)
+    double hwave;        // pi / L: half angles X/2 of the periodic kernels
     double scale;        // 0.5 / L                 (src/compute.cpp:214)
     double R, invR, R2;  // disk radius, 1/R, R^2
-    double G4[kMaxTerms + 1];   // 4 * sum_{m=2..L} exp(-2 pi n m), n = 1..kMaxTerms (velocity)
+    double G4[kMaxTerms + 1];   // sum_{m=2..L} exp(-2 pi n m), n = 1..kMaxTerms (velocity)
     double H4[kMaxTerms + 1];   // 4 * sum_{m=1..L} exp(-2 pi n m) / n (Hamiltonian series)
     double CL;                  // 2 * sum_{m=1..L} log1p(exp(-4 pi m)) (Hamiltonian)
-    double W1;                  // 4 when the rows m = +-1 exist (L >= 1), else 0
+    double W1;                  // 2 when the rows m = +-1 exist (L >= 1), else 0
     double C1, C2, S4;          // cosh 2 pi, 2 cosh 2 pi, 4 sinh^2 2 pi
     float G4f[kMaxTerms + 1];
     float W1f, C1f, C2f, S4f;
@@ -160,84 +161,66 @@ __device__ __forceinline__ double sin_f64(double h, double h2) {
     p = __builtin_fma(p, h2, 1.0);
     return h * p;
 }
-__device__ __forceinline__ double cos_f64(double h2) {
-    double p = 1.0 / 1124000727777607680000.0;             // 1/22!
-    p = __builtin_fma(p, h2, -1.0 / 2432902008176640000.0);  // 1/20!
-    p = __builtin_fma(p, h2, 1.0 / 6402373705728000.0);      // 1/18!
-    p = __builtin_fma(p, h2, -1.0 / 20922789888000.0);       // 1/16!
-    p = __builtin_fma(p, h2, 1.0 / 87178291200.0);           // 1/14!
-    p = __builtin_fma(p, h2, -1.0 / 479001600.0);            // 1/12!
-    p = __builtin_fma(p, h2, 1.0 / 3628800.0);               // 1/10!
-    p = __builtin_fma(p, h2, -1.0 / 40320.0);                // 1/8!
-    p = __builtin_fma(p, h2, 1.0 / 720.0);                   // 1/6!
-    p = __builtin_fma(p, h2, -1.0 / 24.0);                   // 1/4!
-    p = __builtin_fma(p, h2, 0.5);
-    return __builtin_fma(-p, h2, 1.0);
+// 2 cos(h): the Taylor polynomial with every coefficient doubled (exact scalings by 2).
+__device__ __forceinline__ double cos2_f64(double h2) {
+    double p = 2.0 / 1124000727777607680000.0;             // 2/22!
+    p = __builtin_fma(p, h2, -2.0 / 2432902008176640000.0);  // 2/20!
+    p = __builtin_fma(p, h2, 2.0 / 6402373705728000.0);      // 2/18!
+    p = __builtin_fma(p, h2, -2.0 / 20922789888000.0);       // 2/16!
+    p = __builtin_fma(p, h2, 2.0 / 87178291200.0);           // 2/14!
+    p = __builtin_fma(p, h2, -2.0 / 479001600.0);            // 2/12!
+    p = __builtin_fma(p, h2, 2.0 / 3628800.0);               // 2/10!
+    p = __builtin_fma(p, h2, -2.0 / 40320.0);                // 2/8!
+    p = __builtin_fma(p, h2, 2.0 / 720.0);                   // 2/6!
+    p = __builtin_fma(p, h2, -2.0 / 24.0);                   // 2/4!
+    p = __builtin_fma(p, h2, 1.0);                           // 2/2!
+    return __builtin_fma(-p, h2, 2.0);
 }
-// sinh(h) and sin(h) together: with z = h^2, sinh(h) = h P(z) and sin(h) = h P(-z), where
+// 2 sinh(h) and 2 sin(h) together: with z = h^2, sinh(h) = h P(z) and sin(h) = h P(-z), where
 // P(z) = sum_k z^k/(2k+1)!.  Splitting P into even and odd powers, P(+-z) = A(z^2) +- z B(z^2),
-// evaluates both in one pass (A and B are Horner polynomials in z^2 with the same 1/(2k+1)!
-// coefficients as sinh_f64/sin_f64, so the truncation is the same).  On |h| <= pi/2 the sin
-// branch A - z B stays above 2/pi: no cancellation.
-__device__ __forceinline__ void sinh_sin_f64(double h, double h2, double &sh, double &sn) {
+// evaluates both in one pass (A and B are Horner polynomials in z^2 with the coefficients
+// 2/(2k+1)!, the same truncation as sinh_f64/sin_f64 scaled by 2, which is exact).  On
+// |h| <= pi/2 the sin branch A - z B stays above 4/pi: no cancellation.
+__device__ __forceinline__ void sinh_sin2_f64(double h, double h2, double &sh, double &sn) {
     const double w = h2 * h2;
-    double a = 1.0 / 51090942171709440000.0;              // 1/21!
-    a = __builtin_fma(a, w, 1.0 / 355687428096000.0);      // 1/17!
-    a = __builtin_fma(a, w, 1.0 / 6227020800.0);           // 1/13!
-    a = __builtin_fma(a, w, 1.0 / 362880.0);               // 1/9!
-    a = __builtin_fma(a, w, 1.0 / 120.0);                  // 1/5!
-    a = __builtin_fma(a, w, 1.0);                          // 1/1!
-    double b = 1.0 / 121645100408832000.0;                 // 1/19!
-    b = __builtin_fma(b, w, 1.0 / 1307674368000.0);        // 1/15!
-    b = __builtin_fma(b, w, 1.0 / 39916800.0);             // 1/11!
-    b = __builtin_fma(b, w, 1.0 / 5040.0);                 // 1/7!
-    b = __builtin_fma(b, w, 1.0 / 6.0);                    // 1/3!
+    double a = 2.0 / 51090942171709440000.0;              // 2/21!
+    a = __builtin_fma(a, w, 2.0 / 355687428096000.0);      // 2/17!
+    a = __builtin_fma(a, w, 2.0 / 6227020800.0);           // 2/13!
+    a = __builtin_fma(a, w, 2.0 / 362880.0);               // 2/9!
+    a = __builtin_fma(a, w, 2.0 / 120.0);                  // 2/5!
+    a = __builtin_fma(a, w, 2.0);                          // 2/1!
+    double b = 2.0 / 121645100408832000.0;                 // 2/19!
+    b = __builtin_fma(b, w, 2.0 / 1307674368000.0);        // 2/15!
+    b = __builtin_fma(b, w, 2.0 / 39916800.0);             // 2/11!
+    b = __builtin_fma(b, w, 2.0 / 5040.0);                 // 2/7!
+    b = __builtin_fma(b, w, 2.0 / 6.0);                    // 2/3!
     const double zb = h2 * b;
     sh = h * (a + zb);
     sn = h * (a - zb);
 }
-__device__ __forceinline__ void sinh_sin_f32(float h, float h2, float &sh, float &sn) {
+// FP32 forms of sinh_sin2_f64 and cos2_f64 (truncation below 1e-9 relative).
+__device__ __forceinline__ void sinh_sin2_f32(float h, float h2, float &sh, float &sn) {
     const float w = h2 * h2;
-    float a = 1.0f / 6227020800.0f;                        // 1/13!
-    a = __builtin_fmaf(a, w, 1.0f / 362880.0f);            // 1/9!
-    a = __builtin_fmaf(a, w, 1.0f / 120.0f);               // 1/5!
-    a = __builtin_fmaf(a, w, 1.0f);
-    float b = 1.0f / 39916800.0f;                          // 1/11!
-    b = __builtin_fmaf(b, w, 1.0f / 5040.0f);              // 1/7!
-    b = __builtin_fmaf(b, w, 1.0f / 6.0f);                 // 1/3!
+    float a = 2.0f / 6227020800.0f;                        // 2/13!
+    a = __builtin_fmaf(a, w, 2.0f / 362880.0f);            // 2/9!
+    a = __builtin_fmaf(a, w, 2.0f / 120.0f);               // 2/5!
+    a = __builtin_fmaf(a, w, 2.0f);
+    float b = 2.0f / 39916800.0f;                          // 2/11!
+    b = __builtin_fmaf(b, w, 2.0f / 5040.0f);              // 2/7!
+    b = __builtin_fmaf(b, w, 2.0f / 6.0f);                 // 2/3!
     const float zb = h2 * b;
     sh = h * (a + zb);
     sn = h * (a - zb);
 }
-__device__ __forceinline__ float sinh_f32(float h, float h2) {
-    float p = 1.0f / 6227020800.0f;
-    p = __builtin_fmaf(p, h2, 1.0f / 39916800.0f);
-    p = __builtin_fmaf(p, h2, 1.0f / 362880.0f);
-    p = __builtin_fmaf(p, h2, 1.0f / 5040.0f);
-    p = __builtin_fmaf(p, h2, 1.0f / 120.0f);
-    p = __builtin_fmaf(p, h2, 1.0f / 6.0f);
+__device__ __forceinline__ float cos2_f32(float h2) {
+    float p = 2.0f / 87178291200.0f;
+    p = __builtin_fmaf(p, h2, -2.0f / 479001600.0f);
+    p = __builtin_fmaf(p, h2, 2.0f / 3628800.0f);
+    p = __builtin_fmaf(p, h2, -2.0f / 40320.0f);
+    p = __builtin_fmaf(p, h2, 2.0f / 720.0f);
+    p = __builtin_fmaf(p, h2, -2.0f / 24.0f);
     p = __builtin_fmaf(p, h2, 1.0f);
-    return h * p;
-}
-__device__ __forceinline__ float sin_f32(float h, float h2) {
-    float p = 1.0f / 6227020800.0f;
-    p = __builtin_fmaf(p, h2, -1.0f / 39916800.0f);
-    p = __builtin_fmaf(p, h2, 1.0f / 362880.0f);
-    p = __builtin_fmaf(p, h2, -1.0f / 5040.0f);
-    p = __builtin_fmaf(p, h2, 1.0f / 120.0f);
-    p = __builtin_fmaf(p, h2, -1.0f / 6.0f);
-    p = __builtin_fmaf(p, h2, 1.0f);
-    return h * p;
-}
-__device__ __forceinline__ float cos_f32(float h2) {
-    float p = 1.0f / 87178291200.0f;
-    p = __builtin_fmaf(p, h2, -1.0f / 479001600.0f);
-    p = __builtin_fmaf(p, h2, 1.0f / 3628800.0f);
-    p = __builtin_fmaf(p, h2, -1.0f / 40320.0f);
-    p = __builtin_fmaf(p, h2, 1.0f / 720.0f);
-    p = __builtin_fmaf(p, h2, -1.0f / 24.0f);
-    p = __builtin_fmaf(p, h2, 0.5f);
-    return __builtin_fmaf(-p, h2, 1.0f);
+    return __builtin_fmaf(-p, h2, 2.0f);
 }
 
 // ---------------------------------------------------------------- infinite plane
@@ -743,27 +726,24 @@ struct PerF64 {
     __device__ static void zero(Acc &a) { a.u = a.v = 0.0; }
     __device__ static void terms(const GeoParams &p, const Tgt &t, const Src &s, double &fu,
                                  double &fv, bool self) {
-        const double X = p.wave * remainder_fast(t.x - s.x, p.L, p.invL);
-        const double Y = p.wave * remainder_fast(t.y - s.y, p.L, p.invL);
-        const double hx = 0.5 * X, hy = 0.5 * Y;
+        const double hx = p.hwave * remainder_fast(t.x - s.x, p.L, p.invL);   // X / 2
+        const double hy = p.hwave * remainder_fast(t.y - s.y, p.L, p.invL);
         const double hx2 = hx * hx, hy2 = hy * hy;
-        double shx, shy, snx, sny;
-        sinh_sin_f64(hx, hx2, shx, snx);
-        sinh_sin_f64(hy, hy2, shy, sny);
-        const double csx = cos_f64(hx2), csy = cos_f64(hy2);
+        double shx, shy, snx, sny;                     // 2 sinh(X/2), 2 sin(X/2)
+        sinh_sin2_f64(hx, hx2, shx, snx);
+        sinh_sin2_f64(hy, hy2, shy, sny);
+        const double sin_x = snx * cos2_f64(hx2);      // 2 sin X
+        const double sin_y = sny * cos2_f64(hy2);
         const double snx2 = snx * snx, sny2 = sny * sny;
         const double shx2 = shx * shx, shy2 = shy * shy;
-        const double sin_x = 2.0 * snx * csx;
-        const double sin_y = 2.0 * sny * csy;
-        const double cos2x = __builtin_fma(-4.0, snx2, 2.0);   // 2 cos X
-        const double cos2y = __builtin_fma(-4.0, sny2, 2.0);
-        const double ch2x = __builtin_fma(4.0, shx2, 2.0);     // 2 cosh X
-        const double ch2y = __builtin_fma(4.0, shy2, 2.0);
-        const double d0u = self ? 1.0 : 2.0 * (shx2 + sny2);
-        const double d0v = self ? 1.0 : 2.0 * (shy2 + snx2);
-        // Rows |m| >= 2: sin(nY) cosh(nX) and sin(nX) cosh(nY), n = 1..kImageTermsF64.
+        const double cos2x = 2.0 - snx2, cos2y = 2.0 - sny2;   // 2 cos X
+        const double ch2x = 2.0 + shx2, ch2y = 2.0 + shy2;     // 2 cosh X
+        const double du = self ? 1.0 : shx2 + sny2;             // 2 d0
+        const double dv = self ? 1.0 : shy2 + snx2;
+        // Rows |m| >= 2: 2 sin(nY) and 2 cosh(nX) by their recurrences, n = 1..kImageTermsF64
+        // (G4 holds G_n, so G_n (2 sin nY)(2 cosh nX) is the row sum's term n).
         double sy0 = 0.0, sy1 = sin_y, sx0 = 0.0, sx1 = sin_x;
-        double cx0 = 1.0, cx1 = 0.5 * ch2x, cy0 = 1.0, cy1 = 0.5 * ch2y;
+        double cx0 = 2.0, cx1 = ch2x, cy0 = 2.0, cy1 = ch2y;
         double au = 0.0, av = 0.0;
 #pragma unroll
         for (int n = 1; n <= kImageTermsF64; ++n) {
@@ -778,17 +758,18 @@ struct PerF64 {
                 cx0 = cx1, cx1 = cx2, cy0 = cy1, cy1 = cy2;
             }
         }
-        // Rows 0 and +-1: sin Y (q + W1 t d0) / (d0 q).
+        // Rows 0 and +-1: sin Y / d0 + 4 sin Y t / q = (2 sin Y) (q + W1 t (2 d0)) / ((2 d0) q),
+        // W1 = 2.
         const double tu = __builtin_fma(p.C1, ch2x, -cos2y);
         const double tv = __builtin_fma(p.C1, ch2y, -cos2x);
         const double qu = __builtin_fma(ch2x, __builtin_fma(-p.C2, cos2y, ch2x),
                                         __builtin_fma(cos2y, cos2y, p.S4));
         const double qv = __builtin_fma(ch2y, __builtin_fma(-p.C2, cos2x, ch2y),
                                         __builtin_fma(cos2x, cos2x, p.S4));
-        const double nu = __builtin_fma(p.W1 * tu, d0u, qu);
-        const double nv = __builtin_fma(p.W1 * tv, d0v, qv);
-        fu = __builtin_fma(sin_y * nu, rcp64(d0u * qu), au);
-        fv = __builtin_fma(sin_x * nv, rcp64(d0v * qv), av);
+        const double nu = __builtin_fma(p.W1 * tu, du, qu);
+        const double nv = __builtin_fma(p.W1 * tv, dv, qv);
+        fu = __builtin_fma(sin_y * nu, rcp64(du * qu), au);
+        fv = __builtin_fma(sin_x * nv, rcp64(dv * qv), av);
     }
     __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
         double fu, fv;
@@ -856,25 +837,21 @@ struct PerMixed {
     }
     __device__ static void terms(const GeoParams &p, const Tgt &t, const Src &s, float &fu,
                                  float &fv, bool self) {
-        const float X = (float)(p.wave * remainder_fast(t.x - s.x, p.L, p.invL));
-        const float Y = (float)(p.wave * remainder_fast(t.y - s.y, p.L, p.invL));
-        const float hx = 0.5f * X, hy = 0.5f * Y;
+        const float hx = (float)(p.hwave * remainder_fast(t.x - s.x, p.L, p.invL));
+        const float hy = (float)(p.hwave * remainder_fast(t.y - s.y, p.L, p.invL));
         const float hx2 = hx * hx, hy2 = hy * hy;
         float shx, shy, snx, sny;
-        sinh_sin_f32(hx, hx2, shx, snx);
-        sinh_sin_f32(hy, hy2, shy, sny);
-        const float csx = cos_f32(hx2), csy = cos_f32(hy2);
+        sinh_sin2_f32(hx, hx2, shx, snx);
+        sinh_sin2_f32(hy, hy2, shy, sny);
+        const float sin_x = snx * cos2_f32(hx2), sin_y = sny * cos2_f32(hy2);
         const float snx2 = snx * snx, sny2 = sny * sny;
         const float shx2 = shx * shx, shy2 = shy * shy;
-        const float sin_x = 2.f * snx * csx, sin_y = 2.f * sny * csy;
-        const float cos2x = __builtin_fmaf(-4.f, snx2, 2.f);
-        const float cos2y = __builtin_fmaf(-4.f, sny2, 2.f);
-        const float ch2x = __builtin_fmaf(4.f, shx2, 2.f);
-        const float ch2y = __builtin_fmaf(4.f, shy2, 2.f);
-        const float d0u = self ? 1.f : 2.f * (shx2 + sny2);
-        const float d0v = self ? 1.f : 2.f * (shy2 + snx2);
+        const float cos2x = 2.f - snx2, cos2y = 2.f - sny2;
+        const float ch2x = 2.f + shx2, ch2y = 2.f + shy2;
+        const float du = self ? 1.f : shx2 + sny2;
+        const float dv = self ? 1.f : shy2 + snx2;
         float sy0 = 0.f, sy1 = sin_y, sx0 = 0.f, sx1 = sin_x;
-        float cx0 = 1.f, cx1 = 0.5f * ch2x, cy0 = 1.f, cy1 = 0.5f * ch2y;
+        float cx0 = 2.f, cx1 = ch2x, cy0 = 2.f, cy1 = ch2y;
         float au = 0.f, av = 0.f;
 #pragma unroll
         for (int n = 1; n <= kImageTermsF32; ++n) {
@@ -895,10 +872,10 @@ struct PerMixed {
                                         __builtin_fmaf(cos2y, cos2y, p.S4f));
         const float qv = __builtin_fmaf(ch2y, __builtin_fmaf(-p.C2f, cos2x, ch2y),
                                         __builtin_fmaf(cos2x, cos2x, p.S4f));
-        const float nu = __builtin_fmaf(p.W1f * tu, d0u, qu);
-        const float nv = __builtin_fmaf(p.W1f * tv, d0v, qv);
-        fu = __builtin_fmaf(sin_y * nu, rcp32(d0u * qu), au);
-        fv = __builtin_fmaf(sin_x * nv, rcp32(d0v * qv), av);
+        const float nu = __builtin_fmaf(p.W1f * tu, du, qu);
+        const float nv = __builtin_fmaf(p.W1f * tv, dv, qv);
+        fu = __builtin_fmaf(sin_y * nu, rcp32(du * qu), au);
+        fv = __builtin_fmaf(sin_x * nv, rcp32(dv * qv), av);
     }
     __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
         float fu, fv;

@@ -84,20 +84,21 @@ inline pv::GeoParams geo_params(const pv_config &c) {
         p.L = c.box_length_x;
         p.invL = 1.0 / c.box_length_x;
         p.wave = 2.0 * pv::kPi / c.box_length_x;
+        p.hwave = 0.5 * p.wave;
         p.scale = 0.5 / c.box_length_x;
         const long double pi = 3.141592653589793238462643383279502884L;
         for (int n = 1; n <= pv::kMaxTerms; ++n) {
             long double s = 0.0L;
             for (int m = c.image_layers; m >= 2; --m)
                 s += expl(-2.0L * pi * n * m);
-            p.G4[n] = (double)(4.0L * s);
-            p.G4f[n] = (float)(4.0L * s);
+            p.G4[n] = (double)s;
+            p.G4f[n] = (float)s;
             if (c.image_layers >= 1)
                 s += expl(-2.0L * pi * n);
             p.H4[n] = (double)(4.0L * s / n);
         }
         const long double ch = coshl(2.0L * pi), sh = sinhl(2.0L * pi);
-        p.W1 = c.image_layers >= 1 ? 4.0 : 0.0;
+        p.W1 = c.image_layers >= 1 ? 2.0 : 0.0;
         p.C1 = (double)ch;
         p.C2 = (double)(2.0L * ch);
         p.S4 = (double)(4.0L * sh * sh);
