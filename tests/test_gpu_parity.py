@@ -88,6 +88,26 @@ def test_close_encounters():
         kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0, periodicImageLayers=65)
 
 
+@pytest.mark.parametrize("symmetric", [0, 1])
+@pytest.mark.parametrize("precision,seps,bound", [
+    ("fp64", [1e-2, 1e-6, 1e-12, 1e-50, 1e-150], 1e-10),
+    ("mixed", [1e-2, 1e-6, 1e-12], 1e-5)])
+def test_periodic_close_pairs(oracle, precision, seps, bound, symmetric):
+    """A pair at separations down to 1e-150 (FP64) in a population, on both paths.  The rows
+    m = 0, +-1 share one denominator d0 q, and the near-singular m = 0 term must keep its
+    accuracy through it (per-vortex gate of the module docstring; mixed 1e-5)."""
+    pop = wl.periodic(300, 11, box_length=2.0)
+    k = kernel_for("periodic", precision, boxLengthX=2.0, boxLengthY=2.0)
+    k.set_launch(symmetric=symmetric)
+    for sep in seps:
+        x, y = pop.x.copy(), pop.y.copy()
+        x[:2] = [0.0, 0.8 * sep]
+        y[:2] = [0.0, 0.6 * sep]
+        vel = k.evaluate(VortexSystem(x, y, pop.circulation))
+        ur, vr = oracle.evaluate(spec("periodic", box_length=2.0), x, y, pop.circulation)
+        assert per_vortex(vel.x, vel.y, ur, vr) <= bound, sep
+
+
 @pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
 def test_hamiltonian_gradient_and_conservation(geometry):
     """src/audit_tests.cpp:51-85: velocity = (1/G)(dH/dy, -dH/dx) by central differences, and
