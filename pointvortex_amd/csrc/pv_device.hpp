@@ -705,8 +705,9 @@ struct DiskMixed {
 // Rows |m| >= 2: for |X| < 4 pi the exact expansion
 //   sum_{|m| >= 2} ... = 4 sum_{n>=1} G_n sin(nY) cosh(nX),  G_n = sum_{m=2..L} e^{-2 pi n m},
 // evaluated with the Chebyshev recurrences for sin(nY) and cosh(nX).  Term n is at most
-// 4 n e^{-4 pi n} cosh(n pi) |sin Y| times the rows above, so 4 terms leave 2e-19 of the
-// result (FP64) and 2 terms 4e-11 (mixed).  The reference drops image terms with
+// 4 n e^{-4 pi n} cosh(n pi) |sin Y| (|sin nY| <= n |sin Y|), and the m = 0 row is at least
+// |sin Y| / (cosh pi + 1), so 4 terms leave below 5e-19 of the result (FP64) and 2 terms
+// below 4e-11 (mixed).  The reference drops image terms with
 // |X - 2 pi m| > 40 (below 5e-18 relative); the closed form and the series keep them.
 struct PerF64 {
     struct Src {
