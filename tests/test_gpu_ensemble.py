@@ -60,6 +60,25 @@ def test_ensemble_evaluate_matches_oracle(oracle, precision, tol, mode):
         assert normwise(u, v, ur, vr) <= gate, (c, p.n)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-6)])
+def test_ensemble_periodic_layers_per_instance(oracle, precision, tol, mode):
+    """Each instance carries its own image-layer count, and with it its own closed-form and
+    series weights: L = 0, 1, 2, 3, 8, 20 side by side, in two box lengths."""
+    cfgs, pops = [], []
+    for k, layers in enumerate([0, 1, 2, 3, 8, 20] * 2):
+        box = 1.0 if k < 6 else 2.5
+        pops.append(wl.periodic(60 + 20 * k, 300 + k, box_length=box))
+        cfgs.append(InstanceConfig("periodic", box_length=box, image_layers=layers))
+    e = Ensemble(sum(p.n for p in pops), 16, precision)
+    e.set_symmetric(mode)
+    e.load(cfgs, [p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
+    us, vs = e.evaluate()
+    for c, p, u, v in zip(cfgs, pops, us, vs):
+        ur, vr = oracle.evaluate(ospec(c), p.x, p.y, p.circulation)
+        assert normwise(u, v, ur, vr) <= tol, (c, p.n)
+
+
 def test_ensemble_rk4_graph_matches_eager_and_oracle(oracle):
     cfgs, pops = mixed_population(24, 2)
     args = ([p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
