@@ -11,6 +11,8 @@
 //   kBatched        FP64 policies split pair() into prep() (the pair's denominators) and
 //                   apply() (given their inverses), so that the kernel inverts all the
 //                   denominators of a group of pairs with one rcp64 (TreeInverse).
+//   kRingUnroll     unroll of the symmetric ring's step loop (measured per policy: 2 for the
+//                   FP64 and disk policies, 1 for the FP32-heavy infinite and periodic ones).
 //
 // The pair formulas restate src/compute.cpp:117-126 (infinite), 223-249 (periodic) and
 // 306-329 (disk).  The periodic image sum uses the exact Fourier form of the reference's
@@ -243,6 +245,7 @@ struct InfF64 {
     static constexpr bool kBatched = true;
     static constexpr bool kNearTiles = false;
     static constexpr bool kSymNear = false;
+    static constexpr int kRingUnroll = 2;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, kInvTwoPi * g, 0.0};
     }
@@ -325,6 +328,7 @@ struct InfMixed {
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
     static constexpr bool kSymNear = false;
+    static constexpr int kRingUnroll = 1;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
     }
@@ -414,6 +418,7 @@ struct DiskF64 {
     static constexpr bool kBatched = true;
     static constexpr bool kNearTiles = true;
     static constexpr bool kSymNear = false;
+    static constexpr int kRingUnroll = 2;
     __device__ static bool is_near(const GeoParams &p, double x, double y) {
         return __builtin_fma(x, x, y * y) < 0x1p-200 * p.R2;
     }
@@ -568,6 +573,7 @@ struct DiskMixed {
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
     static constexpr bool kSymNear = true;
+    static constexpr int kRingUnroll = 2;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         const double r2 = __builtin_fma(x, x, y * y);
         float ix = 1e30f, iy = 0.f;
@@ -723,6 +729,7 @@ struct PerF64 {
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
     static constexpr bool kSymNear = false;
+    static constexpr int kRingUnroll = 2;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, p.scale * g, 0.0};
     }
@@ -830,6 +837,7 @@ struct PerMixed {
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
     static constexpr bool kSymNear = false;
+    static constexpr int kRingUnroll = 1;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
     }

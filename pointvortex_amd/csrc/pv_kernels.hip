@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
                     r.u = 0;
                     r.v = 0;
                     typename P::Src next = sh[g0 + lane];
-#pragma unroll 2
+#pragma unroll P::kRingUnroll
                     for (int s = 0; s < 64; ++s) {
                         // The next step's source is read one step ahead (at s = 63 this reads
                         // the lane's first source again, unused).
