@@ -294,7 +294,7 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
                     r.u = 0;
                     r.v = 0;
                     typename P::Src next = sl->src[g0 + lane];
-#pragma unroll 2
+#pragma unroll P::kRingUnroll
                     for (int s = 0; s < 64; ++s) {
                         const typename P::Src s_a = next;
                         next = sl->src[g0 + ((lane + dir * (s + 1)) & 63)];
