@@ -10,7 +10,8 @@ reference result), then for every distinct candidate shape:
   product's own bar -- normwise 1e-12 in FP64; in mixed precision, within 1e-6 of the FP64
   kernel and no more than 1.5x the built-in mixed shape's own distance from it;
 * then timing: kernel time from HIP events on the handle's stream, averaged over enough
-  launches to cover about 30 ms.
+  launches to cover about 30 ms.  The shapes within 5 % of the fastest are then timed three
+  more times, in alternation, and the median of their four timings decides.
 
 Among the shapes that passed the gate, the one with the fewest source splits within
 `TIE` (2 %) of the fastest wins: each extra split writes and re-reads one partial sum per
@@ -49,6 +50,8 @@ TABLE = Path(__file__).resolve().parent / "tuning" / "launch_table.txt"
 FP64_GATE = 1e-12
 MIXED_GATE = 1e-6
 TIE = 0.02
+CONFIRM = 0.05
+CONFIRM_ROUNDS = 3
 
 
 def _log(*a):
@@ -130,6 +133,7 @@ def tune_one(geometry, precision, n, log=_log):
             rejected += 1
             continue
         timed[shape] = _time(k)
+    _confirm(k, timed, n)
     k.close()
     fastest = min(timed.values())
     near = [s for s, ms in timed.items() if ms <= fastest * (1.0 + TIE)]
@@ -144,6 +148,29 @@ def tune_one(geometry, precision, n, log=_log):
         f"{default_ms:.4f} ms -> {best_shape} {best_ms:.4f} ms "
         f"({len(seen)} shapes, {rejected} failed the gate)")
     return res
+
+
+def _confirm(k, timed, n):
+    """One 30 ms timing per shape carries a few per cent of noise (clocks, the order in which
+    the shapes ran), enough to turn the tie rule.  The shapes within CONFIRM of the fastest
+    are timed CONFIRM_ROUNDS more times, in alternation, and their medians replace the first
+    timings."""
+    fastest = min(timed.values())
+    close = []
+    for s, ms in timed.items():
+        if ms <= fastest * (1.0 + CONFIRM):
+            k.set_launch(*s)
+            if k.resolved_launch(n, n) == s:   # a shape that can be set again as it ran
+                close.append(s)
+    if len(close) < 2:
+        return
+    samples = {s: [timed[s]] for s in close}
+    for _ in range(CONFIRM_ROUNDS):
+        for s in close:
+            k.set_launch(*s)
+            samples[s].append(_time(k))
+    for s in close:
+        timed[s] = float(np.median(samples[s]))
 
 
 def partial_slots(shape, n):
