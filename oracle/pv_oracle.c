@@ -422,7 +422,7 @@ static const double kDopriA[7][7] = {
     {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0},
     {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0},
     {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
-/* Fifth- and fourth-order weights (src/timestep.cpp:104-108). */
+/* Fifth- and fourth-order weights (src/timestep.cpp:69-73). */
 static const double kB5[7] = {35.0 / 384.0,     0.0, 500.0 / 1113.0, 125.0 / 192.0,
                               -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
 static const double kB4[7] = {5179.0 / 57600.0, 0.0,           7571.0 / 16695.0, 393.0 / 640.0,

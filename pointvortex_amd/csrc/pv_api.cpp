@@ -30,7 +30,7 @@ namespace {
 
 using namespace pv::host;
 
-// src/timestep.cpp:104-108 (b5, b4) and src/timestep.h:8-15 (a_{s,j}).
+// src/timestep.cpp:69-73 (b5, b4) and src/timestep.h:8-15 (a_{s,j}).
 constexpr double kB5[7] = {35.0 / 384.0,     0.0, 500.0 / 1113.0, 125.0 / 192.0,
                            -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
 constexpr double kB4[7] = {5179.0 / 57600.0, 0.0,           7571.0 / 16695.0, 393.0 / 640.0,
@@ -220,10 +220,14 @@ void ensure_candidates(pv_handle *h, size_t cap) {
     h->cand_cap = cap;
 }
 
+// Starts a synchronised sequence of launches: the status word is cleared (fail_epoch back to
+// its UINT_MAX sentinel) and the launch epochs restart from 1, so a sequence's epochs never
+// reach the sentinel (sequences are bounded, see kMaxSequenceSteps).
 void begin_sequence(pv_handle *h) {
     pv::launch_reset_status(h->st, h->stream);
     hip_check(hipGetLastError(), "launch HIP status reset");
-    h->epoch_base = h->epoch + 1;
+    h->epoch = 0;
+    h->epoch_base = 1;
     h->records.clear();
     h->barrier_arrivals = 0;
 }
@@ -1060,6 +1064,10 @@ pv_status pv_rk4_step(pv_handle *h, double dt) {
     });
 }
 
+// RK4 steps per synchronised sequence of pv_rk4_advance: four epochs (and evaluation
+// records) per step, far below the 2^32 epoch range.
+constexpr int kMaxSequenceSteps = 1 << 16;
+
 pv_status pv_rk4_advance(pv_handle *h, double dt, int steps) {
     return guard([&] {
         checked(h);
@@ -1071,27 +1079,31 @@ pv_status pv_rk4_advance(pv_handle *h, double dt, int steps) {
             return;
         DeviceGuard dg(h->cfg.device);
         h->fsal_valid = false;
-        begin_sequence(h);
-        double *bx[2] = {h->x, h->xalt}, *by[2] = {h->y, h->yalt};
-        const int blocks = persistent_blocks(h, h->n);
-        if (!blocks || !enqueue_rk4_persistent(h, dt, steps, blocks, bx, by)) {
-            for (int s = 0; s < steps; ++s)
-                enqueue_rk4(h, dt, bx[s & 1], by[s & 1], bx[(s + 1) & 1], by[(s + 1) & 1]);
+        for (int done = 0; done < steps;) {
+            const int count = std::min(steps - done, kMaxSequenceSteps);
+            begin_sequence(h);
+            double *bx[2] = {h->x, h->xalt}, *by[2] = {h->y, h->yalt};
+            const int blocks = persistent_blocks(h, h->n);
+            if (!blocks || !enqueue_rk4_persistent(h, dt, count, blocks, bx, by)) {
+                for (int s = 0; s < count; ++s)
+                    enqueue_rk4(h, dt, bx[s & 1], by[s & 1], bx[(s + 1) & 1], by[(s + 1) & 1]);
+            }
+            try {
+                finish(h, true);
+            } catch (ApiError &e) {
+                // Keep the state of the last good step: the input of the failing one.
+                const unsigned e0 = h->st_host->fail_epoch;
+                const int failed = (int)((e0 - h->epoch_base) / 4);
+                h->x = bx[failed & 1], h->xalt = bx[(failed + 1) & 1];
+                h->y = by[failed & 1], h->yalt = by[(failed + 1) & 1];
+                e.message += " (RK4 step " + std::to_string(done + failed + 1) + " of " +
+                             std::to_string(steps) + ")";
+                throw;
+            }
+            h->x = bx[count & 1], h->xalt = bx[(count + 1) & 1];
+            h->y = by[count & 1], h->yalt = by[(count + 1) & 1];
+            done += count;
         }
-        try {
-            finish(h, true);
-        } catch (ApiError &e) {
-            // Keep the state of the last good step: the input of the failing one.
-            const unsigned e0 = h->st_host->fail_epoch;
-            const int failed = (int)((e0 - h->epoch_base) / 4);
-            h->x = bx[failed & 1], h->xalt = bx[(failed + 1) & 1];
-            h->y = by[failed & 1], h->yalt = by[(failed + 1) & 1];
-            e.message += " (RK4 step " + std::to_string(failed + 1) + " of " +
-                         std::to_string(steps) + ")";
-            throw;
-        }
-        h->x = bx[steps & 1], h->xalt = bx[(steps + 1) & 1];
-        h->y = by[steps & 1], h->yalt = by[(steps + 1) & 1];
     });
 }
 
@@ -1107,13 +1119,13 @@ pv_status pv_dopri5_step(pv_handle *h, double dt, double atol, double rtol, doub
         unsigned rejected = 0;
         DeviceGuard dg(h->cfg.device);
         if (n == 0) {
-            // An empty population has zero error: src/timestep.cpp:140-149.
+            // An empty population has zero error: src/timestep.cpp:104-114.
             const double suggested = std::clamp(dt * 5.0, min_dt, max_dt);
             h->fsal_valid = true;
             *out = {dt, suggested, 0.0, 0u};
             return;
         }
-        // src/timestep.cpp:111-112: FSAL derivative at the current state.
+        // src/timestep.cpp:75-77: FSAL derivative at the current state.
         if (!h->fsal_valid) {
             begin_sequence(h);
             evaluate(h, h->x, h->y, h->g, n, 0, n, store_op(h->kx[0], h->ky[0]));
@@ -1155,7 +1167,7 @@ pv_status pv_dopri5_step(pv_handle *h, double dt, double atol, double rtol, doub
             finish(h, true);
             double error;
             std::memcpy(&error, &h->st_host->err_bits, sizeof error);
-            // src/timestep.cpp:140-153 (controller and acceptance).
+            // src/timestep.cpp:104-118 (controller and acceptance).
             const double factor =
                 error == 0.0 ? 5.0 : std::clamp(0.9 * std::pow(error, -0.2), 0.2, 5.0);
             const double suggested = std::clamp(dt * factor, min_dt, max_dt);
@@ -1494,6 +1506,8 @@ pv_status pv_evaluate_resident(pv_handle *h, int repeats) {
     return guard([&] {
         checked(h);
         require_state(h);
+        if (repeats > kMaxSequenceSteps)
+            raise(PV_ERR_INVALID_ARGUMENT, "too many repeats in one resident evaluation");
         if (h->n == 0 || repeats <= 0)
             return;
         DeviceGuard dg(h->cfg.device);

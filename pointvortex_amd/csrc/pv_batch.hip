@@ -12,7 +12,7 @@
 //   fixed     one dt for every instance (RK4 ensembles, captured in hipGraphs);
 //   adaptive  per-instance DOPRI5: dt from each instance's InstanceStep, the max-norm error
 //             reduced per instance, failures recorded per instance; a controller kernel then
-//             applies src/timestep.cpp:140-153 to every instance on the device.
+//             applies src/timestep.cpp:104-118 to every instance on the device.
 #include "pv_batch.hpp"
 #include "pv_epilogue.hpp"
 #include "pv_sym.hpp"
@@ -40,6 +40,9 @@ __device__ void batch_item(const BatchLaunch &a, const Instance *ins, const Work
     const int gi = off + (valid ? ti : it.tbegin);
     InstanceStep *stp = &a.steps[it.instance];
     FailSink fail{a.st, a.epoch, &stp->failure};
+    fail.evaluated = stp->evaluated;
+    fail.ex = a.x;
+    fail.ey = a.y;
     if (!ADAPT) {
         fail.keep = stp->keep;
         fail.x0 = a.op.x0;
@@ -387,6 +390,9 @@ __global__ __launch_bounds__(kBatchBlock) void batch_sym_reduce_kernel(BatchLaun
         return;
     const Instance ins = a.inst[k];
     FailSink fail{a.st, a.epoch, &stp->failure};
+    fail.evaluated = stp->evaluated;
+    fail.ex = a.x;
+    fail.ey = a.y;
     if (!ADAPT) {
         fail.keep = stp->keep;
         fail.x0 = a.op.x0;
@@ -444,7 +450,7 @@ __global__ void batch_stage1_kernel(StageOp op, const int *owner, const Instance
     op.yn[i] = op.y0[i] + dt * dy;
 }
 
-// src/timestep.cpp:140-153 per instance, then the step landing of src/main.cpp:312-314 and
+// src/timestep.cpp:104-118 per instance, then the step landing of src/main.cpp:312-314 and
 // the end-of-run test of src/main.cpp:341-344.
 __global__ void batch_control_kernel(InstanceStep *steps, int ninst) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;

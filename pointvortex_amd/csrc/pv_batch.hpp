@@ -42,6 +42,7 @@ struct InstanceStep {
     unsigned failure;                 // FailureKind bits (kFailRejected: too many rejections)
     unsigned fail_step;               // fixed-step mode: steps completed before the failing one
     const double *keep[2];            // fixed-step mode: {x, y} before the failing step
+    const double *evaluated[2];       // {x, y} the first failing evaluation read
 };
 
 // Symmetric mode: one unit of work is a pair of kSymTile-vortex tiles (I <= J) of one
@@ -61,6 +62,11 @@ struct PairItem {
 
 enum InstanceState : int { kRunning = 0, kFinished = 1, kFailed = 2, kIdle = 3 };
 constexpr unsigned kFailRejected = 32u;
+// Set by the host once a non-finite velocity of the instance has been classified (the exact
+// zero-denominator test); with kFailCoincident, bits 8-9 carry the instance's geometry so the
+// message names it as the single-system path does.
+constexpr unsigned kFailClassified = 64u;
+constexpr int kFailGeometryShift = 8;
 
 enum BatchPass : int {
     kPassAll = 0,        // every item

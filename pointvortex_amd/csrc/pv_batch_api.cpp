@@ -23,7 +23,7 @@ using namespace pv::host;
 
 namespace {
 
-// src/timestep.h:8-15 and src/timestep.cpp:104-108.
+// src/timestep.h:8-15 and src/timestep.cpp:69-73.
 constexpr double kB5[7] = {35.0 / 384.0,     0.0, 500.0 / 1113.0, 125.0 / 192.0,
                            -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
 constexpr double kB4[7] = {5179.0 / 57600.0, 0.0,           7571.0 / 16695.0, 393.0 / 640.0,
@@ -123,7 +123,9 @@ struct pv_batch {
     size_t total = 0;
     size_t nfixed = 0;
     unsigned long long steps_done = 0;   // RK4 steps since pv_batch_load
-    std::map<std::pair<double, bool>, hipGraphExec_t> graphs;  // (dt, orientation) -> 2 steps
+    // Two captured RK4 steps per (dt, buffer the pair starts from): a graph holds the pointers
+    // it was captured with, and an odd eager tail swaps x and xn.
+    std::map<std::pair<double, const double *>, hipGraphExec_t> graphs;
 
     // adaptive layout
     std::vector<char> used;
@@ -344,6 +346,34 @@ void read_status(pv_batch *b) {
     hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
 }
 
+// A non-finite velocity of an instance is a coincident pair when one of the reference's
+// denominators is exactly zero (src/compute.cpp:119-123, 234-246, 307-313); the single-system
+// path asks classify_kernel after a failure, and so does this, on the positions the failing
+// evaluation read (kept intact: a failed instance is skipped from then on).  The result goes
+// into the instance's failure word, once per failure, with its geometry for the message.
+void classify_failures(pv_batch *b, size_t ninst) {
+    for (size_t k = 0; k < ninst; ++k) {
+        pv::InstanceStep &s = b->steps_h[k];
+        if (!(s.failure & pv::kFailVelocity) || (s.failure & pv::kFailClassified) ||
+            !s.evaluated[0])
+            continue;
+        const pv::Instance &in = b->inst_h[k];
+        reset_status(b);
+        pv::launch_classify(in.gp, s.evaluated[0] + in.offset, s.evaluated[1] + in.offset, in.n,
+                            0, in.n, b->st, b->stream);
+        hip_check(hipGetLastError(), "launch HIP coincidence check");
+        read_status(b);
+        s.failure |= pv::kFailClassified;
+        if (b->st_host->flags & pv::kFailCoincident)
+            s.failure |= pv::kFailCoincident |
+                         ((unsigned)in.gp.geometry << pv::kFailGeometryShift);
+        hip_check(hipMemcpyAsync(&b->steps_d[k].failure, &s.failure, sizeof(unsigned),
+                                 hipMemcpyHostToDevice, b->stream),
+                  "upload instance failure");
+    }
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+}
+
 // Fixed-step calls fail per instance (the instance stops; the others go on): refresh the
 // host mirror of the instance words once the stream is done.
 void finish_fixed(pv_batch *b) {
@@ -352,6 +382,7 @@ void finish_fixed(pv_batch *b) {
                                  hipMemcpyDeviceToHost, b->stream),
                   "read instance states");
     hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    classify_failures(b, b->nfixed);
 }
 
 // Every instance slot idle, no vortex owned, the whole capacity free.
@@ -631,8 +662,10 @@ pv_status pv_batch_rk4(pv_batch *b, double dt, int steps, int use_graph) {
         reset_status(b);
         int done = 0;
         if (use_graph && steps >= 2) {
-            // Two steps per replay: x -> xn -> x, so the captured pointers stay valid.
-            const auto key = std::make_pair(dt, false);
+            // Two steps per replay: x -> xn -> x, so the captured pointers stay valid.  Keyed
+            // on the buffer x is in now: after an odd step count x and xn have traded places,
+            // and a graph captured the other way round would step from the stale buffer.
+            const auto key = std::make_pair(dt, static_cast<const double *>(b->x));
             auto it = b->graphs.find(key);
             if (it == b->graphs.end()) {
                 hipGraph_t graph = nullptr;
@@ -837,7 +870,7 @@ static void enqueue_attempt(pv_batch *b) {
         op.w[s] = kB5[s] - kB4[s];
     }
     op.x0 = b->x, op.y0 = b->y;
-    // FSAL derivative for runs that have none yet (src/timestep.cpp:111-112).
+    // FSAL derivative for runs that have none yet (src/timestep.cpp:75-77).
     op.kind = pv::kOpStore;
     op.ku = b->kx[0], op.kv = b->ky[0];
     launch(b, b->x, b->y, op, true, pv::kPassNeedK0);
@@ -885,6 +918,8 @@ pv_status pv_batch_attempts(pv_batch *b, int count) {
                                  hipMemcpyDeviceToHost, b->stream),
                   "read instance states");
         hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        // Before the run can be released and its vortex range reused.
+        classify_failures(b, b->max_inst);
         for (size_t k = 0; k < b->max_inst; ++k)
             if (b->in_items[k] && b->steps_h[k].state != pv::kRunning)
                 b->items_dirty = true;
@@ -973,6 +1008,13 @@ const char *pv_batch_failure_message(unsigned failure) {
         return "vortex data must be finite";
     if (failure & pv::kFailOutsideDisk)
         return "vortex lies on or outside the disk";
+    if (failure & pv::kFailCoincident) {
+        // The single-system messages (src/compute.cpp:131-132, 254-255, 334-335).
+        const unsigned geometry = (failure >> pv::kFailGeometryShift) & 3u;
+        return geometry == PV_INFINITE   ? "coincident vortices in singular point-vortex kernel"
+               : geometry == PV_PERIODIC ? "coincident periodic vortices"
+                                         : "coincident vortices in disk";
+    }
     if (failure & pv::kFailVelocity)
         return "non-finite velocity; check scales and close encounters";
     if (failure & pv::kFailErrorEstimate)

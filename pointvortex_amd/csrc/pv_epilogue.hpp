@@ -26,13 +26,24 @@ struct FailSink {
     const double *x0 = nullptr, *y0 = nullptr;
     unsigned *steps_at_failure = nullptr;   // receives *steps_done at the first failure
     const unsigned *steps_done = nullptr;
+    // Ensembles: where to record the positions the failing evaluation read, so that the host
+    // can tell a coincident pair from an overflow afterwards (the failing instance is skipped
+    // from then on, so those positions stay as they were).
+    const double **evaluated = nullptr;
+    const double *ex = nullptr, *ey = nullptr;
     __device__ __forceinline__ void operator()(unsigned kind) const {
         if (word) {
             const unsigned old = atomicOr(word, kind);
-            if (old == 0u && keep) {
-                keep[0] = x0;
-                keep[1] = y0;
-                *steps_at_failure = *steps_done;
+            if (old == 0u) {
+                if (evaluated) {
+                    evaluated[0] = ex;
+                    evaluated[1] = ey;
+                }
+                if (keep) {
+                    keep[0] = x0;
+                    keep[1] = y0;
+                    *steps_at_failure = *steps_done;
+                }
             }
         } else {
             record_failure(st, epoch, kind);

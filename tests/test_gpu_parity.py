@@ -340,8 +340,25 @@ def test_dopri5_parity(oracle, geometry):
 
 
 def test_dopri_fsal_counts_and_invalidation(oracle):
-    """FSAL reuse (src/tests.cpp:297-314): after invalidation the first stage is recomputed;
-    results match the oracle's integrator driven the same way."""
+    """src/tests.cpp:297-314: the co-rotating pair takes 7 evaluations in its first DOPRI5
+    step, 13 after the second (the FSAL stage is reused) and 20 after an invalidation (the
+    first stage is recomputed), counted by the handle's launch counter.  On a 64-vortex
+    population the same bookkeeping holds with rejections (6 more per rejected attempt), and
+    the errors match the oracle's integrator driven the same way."""
+    k = kernel_for("infinite")
+    k.uploadDeviceState(VortexSystem([-1.0, 1.0], [0.0, 0.0], [1.0, 1.0]))
+    k.profileBegin()
+    counts = []
+    for step in range(3):
+        if step == 2:
+            k.invalidateDeviceDerivative()
+        r = k.deviceDopri5Step(1e-3, 1e-10, 1e-8, 1e-12, 0.1)
+        assert r.rejectedSteps == 0
+        counts.append(k.profileEnd()[1])
+        k.profileBegin()
+    k.profileEnd()
+    assert np.cumsum(counts).tolist() == [7, 13, 20]
+
     pop = wl.infinite(64, 3)
     k = kernel_for("infinite")
     k.uploadDeviceState(VortexSystem(pop.x, pop.y, pop.circulation))
@@ -352,8 +369,12 @@ def test_dopri_fsal_counts_and_invalidation(oracle):
         if step == 2:
             k.invalidateDeviceDerivative()
             it.invalidate()
+        k.profileBegin()
         rg = k.deviceDopri5Step(1e-3, 1e-10, 1e-8, 1e-12, 0.1)
+        launches = k.profileEnd()[1]
         ro = it.dopri5_step(s, x, y, pop.circulation, 1e-3, 1e-10, 1e-8, 1e-12, 0.1)
+        assert rg.rejectedSteps == ro[3]
+        assert launches == (step != 1) + 6 * (1 + rg.rejectedSteps)
         assert abs(rg.normalizedError - ro[2]) <= 1e-6 * ro[2]
 
 
