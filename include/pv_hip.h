@@ -237,6 +237,23 @@ pv_status pv_profile_end(pv_handle *h, double *kernel_ms, int *launches);
  * kernel time.  Used to calibrate the HBM-traffic counters (FETCH_SIZE, WRITE_SIZE). */
 pv_status pv_calibration_copy(int device, size_t bytes, int repeats, double *ms_per_copy);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the evaluation of
+ * pv_evaluate (VelocityKernel::evaluate, src/compute.cpp:60-69) for a population already in
+ * device memory, allocating nothing.  x, y, gamma (n each) are read and u, v (n each) written;
+ * all five are device pointers.  The split pair kernel's status word, tile tickets and partial
+ * sums live in a fixed device array of PV_NOALLOC_SCRATCH_BYTES that each GPU the library runs
+ * on holds until the process ends.  n is at most PV_NOALLOC_MAX_VORTICES (larger populations
+ * are rejected with PV_ERR_INVALID_ARGUMENT); source splits are capped so that their partial
+ * sums fit.  Runs on `stream` (a hipStream_t; NULL is the null stream) and waits for it.
+ * Validation and messages are those of pv_evaluate.  Calls on one GPU share its array:
+ * calls that overlap on one GPU race. */
+#define PV_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+#define PV_NOALLOC_MAX_VORTICES ((size_t)1 << 22)
+pv_status pv_evaluate_device_noalloc(const pv_config *config, const double *x, const double *y,
+                                     const double *gamma, size_t n, double *u, double *v,
+                                     void *stream);
+
 /* ------------------------------------------------------------------------------------
  * Ensembles (no reference counterpart; each instance is what the reference runs as one
  * process).  Instances of uneven N and any geometry share flat device arrays and advance

@@ -161,6 +161,8 @@ _SIGNATURES = {
     "pv_profile_begin": (ctypes.c_int, [_vp]),
     "pv_profile_end": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "pv_calibration_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp]),
+    "pv_evaluate_device_noalloc": (ctypes.c_int, [ctypes.POINTER(PvConfig), _vp, _vp, _vp,
+                                                  ctypes.c_size_t, _vp, _vp, _vp]),
     "pv_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
                                        ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "pv_batch_destroy": (None, [_vp]),
@@ -537,6 +539,24 @@ def backendRuntimeDetails(device: int = 0) -> str:
     buf = ctypes.create_string_buffer(1024)
     _check(lib.pv_runtime_details(int(device), buf, len(buf)))
     return buf.value.decode()
+
+
+NOALLOC_SCRATCH_BYTES = 16 << 20
+NOALLOC_MAX_VORTICES = 1 << 22
+
+
+def evaluate_device_noalloc(params: SimParams, x_ptr: int, y_ptr: int, g_ptr: int, n: int,
+                            u_ptr: int, v_ptr: int, stream: int = 0):
+    """pv_evaluate_device_noalloc: VelocityKernel::evaluate (src/compute.cpp:60-69) of a
+    population already in device memory (x, y, circulation, u, v are device addresses), with
+    no device allocation: the kernels' scratch is a fixed per-GPU array of the library."""
+    lib = load_library()
+    p = params
+    cfg = PvConfig(GEOMETRIES[p.boundaryCondition], PRECISIONS[p.precision],
+                   float(p.coreRadius), float(p.boxLengthX), float(p.boxLengthY),
+                   int(p.periodicImageLayers), float(p.diskRadius), int(p.device))
+    _check(lib.pv_evaluate_device_noalloc(ctypes.byref(cfg), x_ptr, y_ptr, g_ptr, int(n),
+                                          u_ptr, v_ptr, stream or None))
 
 
 def calibration_copy(nbytes: int, repeats: int = 10, device: int = 0) -> float:

@@ -32,7 +32,8 @@ COMMON = [
     "-ffp-contract=fast-honor-pragmas", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-SOURCES = ["pv_kernels.hip", "pv_batch.hip", "pv_persist.hip", "pv_api.cpp", "pv_batch_api.cpp"]
+SOURCES = ["pv_kernels.hip", "pv_batch.hip", "pv_persist.hip", "pv_api.cpp", "pv_batch_api.cpp",
+           "pv_noalloc.cpp"]
 
 
 def _compile(src):

@@ -263,27 +263,14 @@ Shape resolve_shape(const pv_handle *h, size_t targets, size_t n) {
         tile = h->tile_override;
     if (h->sym_override >= 0)
         sym = h->sym_override;
-    if (tpt <= 0)
-        tpt = targets >= 32768 ? 2 : 1;
-    tpt = tpt >= 4 ? 4 : (tpt >= 2 ? 2 : 1);
-    if (tile != 128 && tile != 256)
-        tile = pv::pair_tile(prec);
+    const SplitShape sp = split_shape(targets, n, tpt, splits, tile, prec, h->cu_count);
     Shape s{};
-    s.tpt = tpt;
-    s.tile = tile;
-    s.tiles = (targets + (size_t)pv::kBlock * tpt - 1) / ((size_t)pv::kBlock * tpt);
-    const size_t max_split = std::max<size_t>(1, (n + tile - 1) / tile);
-    size_t nsplit = splits > 0 ? (size_t)splits : 0;
-    if (nsplit == 0) {
-        const size_t want = (size_t)4 * h->cu_count;
-        nsplit = std::max<size_t>(1, (want + s.tiles - 1) / std::max<size_t>(1, s.tiles));
-    }
-    nsplit = std::min(nsplit, max_split);
-    size_t split_len = (n + nsplit - 1) / nsplit;
-    split_len = std::max<size_t>(tile, (split_len + tile - 1) / tile * tile);
-    s.split_len = split_len;
-    s.nsplit = std::max<size_t>(1, (n + split_len - 1) / split_len);
-    const size_t tt = (size_t)pv::kBlock * tpt, nt = (n + tt - 1) / tt;
+    s.tpt = sp.tpt;
+    s.tile = sp.tile;
+    s.tiles = sp.tiles;
+    s.split_len = sp.split_len;
+    s.nsplit = sp.nsplit;
+    const size_t tt = (size_t)pv::kBlock * s.tpt, nt = (n + tt - 1) / tt;
     s.sym = sym == 1 && pv::sym_supported(h->cfg.geometry, prec) && targets == n && n > 0 &&
             nt * 2 * n * sizeof(double) <= kSymPartialLimit;
     return s;

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 
@@ -116,6 +117,40 @@ inline pv::GeoParams geo_params(const pv_config &c) {
         p.R2 = c.disk_radius * c.disk_radius;
     }
     return p;
+}
+
+// Launch geometry of the split pair kernel for `targets` targets against n sources.  tpt,
+// splits and tile of 0 are automatic: 2 targets per thread from 32768 targets, the precision's
+// LDS tile, and enough source splits for about 4 blocks per CU; the splits are further capped
+// at max_splits (a bound on the partial-sum buffer) and at one LDS tile of sources each.
+struct SplitShape {
+    int tpt, tile;
+    size_t tiles, nsplit, split_len;
+};
+
+inline SplitShape split_shape(size_t targets, size_t n, int tpt, int splits, int tile,
+                              int precision, int cu_count, size_t max_splits = SIZE_MAX) {
+    if (tpt <= 0)
+        tpt = targets >= 32768 ? 2 : 1;
+    tpt = tpt >= 4 ? 4 : (tpt >= 2 ? 2 : 1);
+    if (tile != 128 && tile != 256)
+        tile = pv::pair_tile(precision);
+    SplitShape s{};
+    s.tpt = tpt;
+    s.tile = tile;
+    s.tiles = (targets + (size_t)pv::kBlock * tpt - 1) / ((size_t)pv::kBlock * tpt);
+    const size_t max_split = std::max<size_t>(1, (n + tile - 1) / tile);
+    size_t nsplit = splits > 0 ? (size_t)splits : 0;
+    if (nsplit == 0) {
+        const size_t want = (size_t)4 * cu_count;
+        nsplit = std::max<size_t>(1, (want + s.tiles - 1) / std::max<size_t>(1, s.tiles));
+    }
+    nsplit = std::max<size_t>(1, std::min({nsplit, max_split, max_splits}));
+    size_t split_len = (n + nsplit - 1) / nsplit;
+    split_len = std::max<size_t>(tile, (split_len + tile - 1) / tile * tile);
+    s.split_len = split_len;
+    s.nsplit = std::max<size_t>(1, (n + split_len - 1) / split_len);
+    return s;
 }
 
 // Constructor checks of src/compute.cpp:50-56, 165-173, 258-261 and the geometry switch of
