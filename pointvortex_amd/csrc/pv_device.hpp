@@ -40,6 +40,7 @@ struct GeoParams {
     double hwave;        // pi / L: half angles X/2 of the periodic kernels
     double scale;        // 0.5 / L                 (src/compute.cpp:214)
     double R, invR, R2;  // disk radius, 1/R, R^2
+    double R2lo;         // R^2 - R2 exactly (fma(R, R, -R2)): R2 + R2lo is R*R
     double G4[kMaxTerms + 1];   // sum_{m=2..L} exp(-2 pi n m), n = 1..kMaxTerms (velocity)
     double H4[kMaxTerms + 1];   // 4 * sum_{m=1..L} exp(-2 pi n m) / n (Hamiltonian series)
     double CL;                  // 2 * sum_{m=1..L} log1p(exp(-4 pi m)) (Hamiltonian)
@@ -104,8 +105,11 @@ __device__ __forceinline__ bool tree_fallback(bool ok) {
     return __builtin_amdgcn_ballot_w64(!ok) != 0;
 }
 
-// Error-free addition (Knuth's TwoSum): s + e == a + b exactly.
+// Error-free addition (Knuth's TwoSum): s + e == a + b exactly.  Contraction is off here: a
+// caller's product fused into these additions (a + x*y as one fma) would make s exact while e
+// still measures the rounded product, and the pair would no longer be error-free.
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+#pragma clang fp contract(off)
     s = a + b;
     const double bp = s - a;
     e = (a - (s - bp)) + (b - bp);
@@ -401,12 +405,40 @@ struct InfMixed {
 // plane-vortex terms with the image point computed once per staged source.  A source within
 // 2^-100 R of the centre has its image point near overflow; a tile holding one is computed
 // with the reference's a, b form instead (the tile flag is uniform, so no per-pair branch).
+// Offset of the image point x* = x R^2/|x|^2 from x: delta = x (R^2 - |x|^2)/|x|^2.  R^2 - |x|^2
+// is formed from the rounded squares with their exact rounding errors (fma) and exact sums
+// (two_sum), so it keeps a few ulps of relative accuracy however close x is to the wall, and so
+// does delta.  The image difference x_i - x_j* = d - delta_j (d = x_i - x_j) is then accurate to
+// a few ulps as well: for x_i inside the disk, |d| + |delta_j| <= 5 |x_i - x_j*| (|delta_j| <=
+// 2 dist(x_j*, disk) <= 2 |x_i - x_j*|, and |d| <= |x_i - x_j*| + |delta_j|), so the subtraction
+// cannot cancel.  Forming x_j* itself instead rounds it by an ulp of R, a relative error of
+// ulp(R) / |x_i - x_j*| in the image term, about 1e-12 for pairs 1e-4 from the wall.
+// R^2 - |x|^2 to a few ulps, however close x is to the wall.  Contraction is off: R2 - x*x
+// fused into one fma would no longer pair with the fma that measures x*x's rounding.
+__device__ __forceinline__ double wall_gap(const GeoParams &p, double x, double y) {
+#pragma clang fp contract(off)
+    const double px = x * x, ex = __builtin_fma(x, x, -px);
+    const double py = y * y, ey = __builtin_fma(y, y, -py);
+    double s1, e1, s2, e2;
+    two_sum(p.R2, -px, s1, e1);
+    two_sum(s1, -py, s2, e2);
+    return s2 + (((e1 + e2) + p.R2lo) - (ex + ey));
+}
+
+__device__ __forceinline__ void image_offset(const GeoParams &p, double x, double y, double r2,
+                                             double &ox, double &oy) {
+    const double q = wall_gap(p, x, y) / r2;
+    ox = x * q;
+    oy = y * q;
+}
+
 struct DiskF64 {
-    // g = G/(2 pi); (ix, iy) the image point x R^2/|x|^2 (strength -g); w = 1/|x|^2.  A source
-    // within 2^-100 R of the centre has no finite image: (ix, iy, w) = 0, and every tile that
-    // holds one is evaluated in the reference's form instead.
+    // g = G/(2 pi); (ox, oy) = delta, the image point's offset x* - x (image_offset; the image
+    // has strength -g); w = 1/|x|^2.  A source within 2^-100 R of the centre has no finite
+    // image: (ox, oy, w) = 0, and every tile that holds one is evaluated in the reference's
+    // form instead.
     struct Src {
-        double x, y, g, ix, iy, w;
+        double x, y, g, ox, oy, w;
     };
     struct Tgt {
         double x, y, tx, ty;
@@ -427,8 +459,9 @@ struct DiskF64 {
         const double gg = kInvTwoPi * g;
         if (r2 < 0x1p-200 * p.R2)
             return {x, y, gg, 0.0, 0.0, 0.0};
-        const double q = p.R2 / r2;
-        return {x, y, gg, x * q, y * q, 1.0 / r2};
+        double ox, oy;
+        image_offset(p, x, y, r2, ox, oy);
+        return {x, y, gg, ox, oy, 1.0 / r2};
     }
     __device__ static Tgt tgt(const GeoParams &p, double x, double y) {
         return {x, y, x * p.invR, y * p.invR};
@@ -440,8 +473,9 @@ struct DiskF64 {
         a.v = __builtin_fma(c, dx, a.v);
     }
     __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, Acc &a) {
-        point(t.x - s.x, t.y - s.y, s.g, a);
-        point(t.x - s.ix, t.y - s.iy, -s.g, a);
+        const double dx = t.x - s.x, dy = t.y - s.y;
+        point(dx, dy, s.g, a);
+        point(dx - s.ox, dy - s.oy, -s.g, a);
     }
     __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, Acc &a,
                                         bool self) {
@@ -451,7 +485,7 @@ struct DiskF64 {
         const double c = self ? 0.0 : s.g * rcp64(r2);
         a.u = __builtin_fma(-c, dy, a.u);
         a.v = __builtin_fma(c, dx, a.v);
-        point(t.x - s.ix, t.y - s.iy, -s.g, a);
+        point(dx - s.ox, dy - s.oy, -s.g, a);     // a vortex's own image too (d = 0)
     }
     // The reference's form (src/compute.cpp:318-329), for tiles holding a near-centre source.
     __device__ static void pair_ref_checked(const GeoParams &p, const Tgt &t, const Src &s,
@@ -485,8 +519,8 @@ struct DiskF64 {
         m.dx = t.x - s.x;
         m.dy = t.y - s.y;
         d[0] = __builtin_fma(m.dx, m.dx, m.dy * m.dy);
-        m.ex = t.x - s.ix;
-        m.ey = t.y - s.iy;
+        m.ex = m.dx - s.ox;
+        m.ey = m.dy - s.oy;
         d[1] = __builtin_fma(m.ex, m.ex, m.ey * m.ey);
     }
     __device__ static void apply(const Src &s, const Tmp &m, const double *inv, Acc &a) {
@@ -501,8 +535,9 @@ struct DiskF64 {
     // still gets its own image term (j's image at i, i's image at j).  The two image distances
     // share one reciprocal: |x_i - x_j*|^2 |x_j|^2 = |x_i|^2 |x_j|^2 - 2 R^2 x_i.x_j + R^4 is
     // symmetric in i and j, so 1/|x_j - x_i*|^2 = |x_i|^2 w_j / |x_i - x_j*|^2.  Both
-    // differences are still formed directly (no cancellation); only the reciprocal is shared.
-    // Blocks holding a near-centre vortex use the reference's form both ways.
+    // differences are still formed from the accurate offsets (no cancellation); only the
+    // reciprocal is shared.  Blocks holding a near-centre vortex use the reference's form both
+    // ways.
     static constexpr int kSymDenoms = 2;
     using SymScalar = double;
     using SymR = Acc;
@@ -515,15 +550,15 @@ struct DiskF64 {
     __device__ static double sym_g(const GeoParams &, double g) { return kInvTwoPi * g; }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) {
         const Src s = src(p, x, y, 0.0);
-        return {x, y, s.ix - x, s.iy - y, __builtin_fma(x, x, y * y)};
+        return {x, y, s.ox, s.oy, __builtin_fma(x, x, y * y)};
     }
     __device__ static void sym_prep(const GeoParams &, const SymTgt &t, const Src &s, SymTmp &m,
                                     double *d) {
         m.dx = t.x - s.x;
         m.dy = t.y - s.y;
         d[0] = __builtin_fma(m.dx, m.dx, m.dy * m.dy);
-        m.ex = t.x - s.ix;
-        m.ey = t.y - s.iy;
+        m.ex = m.dx - s.ox;
+        m.ey = m.dy - s.oy;
         d[1] = __builtin_fma(m.ex, m.ex, m.ey * m.ey);
         m.r2 = t.r2;
         m.w = s.w;

@@ -115,6 +115,7 @@ inline pv::GeoParams geo_params(const pv_config &c) {
         p.R = c.disk_radius;
         p.invR = 1.0 / c.disk_radius;
         p.R2 = c.disk_radius * c.disk_radius;
+        p.R2lo = std::fma(c.disk_radius, c.disk_radius, -p.R2);
     }
     return p;
 }

@@ -529,13 +529,33 @@ __global__ void moments_kernel(const double *x, const double *y, const double *g
     }
 }
 
+// Per-source data of the disk Hamiltonian: the image offset delta_j = x_j* - x_j (image_offset,
+// accurate however close x_j is to the wall) and |x_j|^2; zero for a source within 2^-100 R of
+// the centre, whose pairs take the reference's form.
+struct EnergySrc {
+    double ox, oy, r2;
+};
+
+__device__ __forceinline__ EnergySrc energy_src(const GeoParams &gp, double x, double y) {
+    const double r2 = __builtin_fma(x, x, y * y);
+    EnergySrc s{0.0, 0.0, 0.0};
+    if (gp.geometry == 2 && r2 >= 0x1p-200 * gp.R2) {
+        image_offset(gp, x, y, r2, s.ox, s.oy);
+        s.r2 = r2;
+    }
+    return s;
+}
+
 // Pair energy e_ij of the Hamiltonians (symmetric in i, j):
 //   infinite  log(r^2 + eps^2)                                      src/compute.cpp:142-151
 //   disk      log(|d|^2 R^2 / ((R^2 - x_i.x_j)^2 + (x_i x x_j)^2))   src/compute.cpp:271-282
 //   periodic  -X^2/2pi + sum_{|m|<=L} [log D_m - log cosh 2 pi m]   src/compute.cpp:182-199
 // with the periodic rows m != 0 summed exactly as -4 sum_n G_n cos(nY) cosh(nX) / n - C_L.
+// The disk denominator is |x_j|^2 |x_i - x_j*|^2, the same quantity ((R^2 - a.b)^2 + (a x b)^2
+// = R^4 - 2 R^2 a.b + |a|^2 |b|^2), formed from the image offset so that it does not cancel for
+// pairs near the wall as R^2 - x_i.x_j does.
 __device__ double pair_energy(const GeoParams &gp, double xi, double yi, double xj, double yj,
-                              bool &zero) {
+                              const EnergySrc &sj, bool &zero) {
     if (gp.geometry == 0) {
         const double dx = xi - xj, dy = yi - yj;
         const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, gp.eps2));
@@ -545,10 +565,15 @@ __device__ double pair_energy(const GeoParams &gp, double xi, double yi, double 
     if (gp.geometry == 2) {
         const double dx = xi - xj, dy = yi - yj;
         const double num = __builtin_fma(dx, dx, dy * dy) * gp.R2;
-        const double dot = __builtin_fma(xi, xj, yi * yj);
-        const double cross = __builtin_fma(yi, xj, -(xi * yj));
-        const double den = __builtin_fma(gp.R2 - dot, gp.R2 - dot, cross * cross);
         zero = num == 0.0;
+        if (sj.r2 == 0.0) {
+            const double dot = __builtin_fma(xi, xj, yi * yj);
+            const double cross = __builtin_fma(yi, xj, -(xi * yj));
+            const double den = __builtin_fma(gp.R2 - dot, gp.R2 - dot, cross * cross);
+            return log(num) - log(den);
+        }
+        const double ex = dx - sj.ox, ey = dy - sj.oy;
+        const double den = sj.r2 * __builtin_fma(ex, ex, ey * ey);
         return log(num) - log(den);
     }
     const double X = gp.wave * remainder_fast(xi - xj, gp.L, gp.invL);
@@ -578,6 +603,7 @@ __global__ __launch_bounds__(kReduceBlock) void energy_kernel(GeoParams gp, cons
                                                               int n, double *partials,
                                                               DevStatus *st) {
     __shared__ double sx[kReduceBlock], sy[kReduceBlock], sg[kReduceBlock];
+    __shared__ EnergySrc se[kReduceBlock];
     __shared__ double red[kReduceBlock][2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
@@ -589,9 +615,11 @@ __global__ __launch_bounds__(kReduceBlock) void energy_kernel(GeoParams gp, cons
         const int cnt = min(kReduceBlock, n - base);
         __syncthreads();
         if ((int)threadIdx.x < cnt) {
-            sx[threadIdx.x] = x[base + threadIdx.x];
-            sy[threadIdx.x] = y[base + threadIdx.x];
+            const double xj = x[base + threadIdx.x], yj = y[base + threadIdx.x];
+            sx[threadIdx.x] = xj;
+            sy[threadIdx.x] = yj;
             sg[threadIdx.x] = g[base + threadIdx.x];
+            se[threadIdx.x] = energy_src(gp, xj, yj);
         }
         __syncthreads();
         if (valid) {
@@ -599,7 +627,7 @@ __global__ __launch_bounds__(kReduceBlock) void energy_kernel(GeoParams gp, cons
                 if (base + k == i)
                     continue;
                 bool z = false;
-                const double eij = pair_energy(gp, xi, yi, sx[k], sy[k], z);
+                const double eij = pair_energy(gp, xi, yi, sx[k], sy[k], se[k], z);
                 coincident |= z;
                 e.add(sg[k] * eij);
             }
@@ -613,10 +641,12 @@ __global__ __launch_bounds__(kReduceBlock) void energy_kernel(GeoParams gp, cons
         hc.add(-gi / (8.0 * kPi) * e.hi);
         hc.add(-gi / (8.0 * kPi) * e.lo);
         if (gp.geometry == 2) {
+            // Self term G^2 log((R^2 - |x_i|^2)/R) / 4 pi, with R^2 - |x_i|^2 formed without
+            // cancellation near the wall (as in image_offset).
             const double ri2 = xi * xi + yi * yi;
             if (ri2 >= gp.R2)
                 atomicOr(&st->flags, (unsigned)kFailOutsideDisk);
-            hc.add(gi * gi * log((gp.R2 - ri2) / gp.R) / (4.0 * kPi));
+            hc.add(gi * gi * log(wall_gap(gp, xi, yi) / gp.R) / (4.0 * kPi));
         }
         if (coincident)
             atomicOr(&st->flags, (unsigned)kFailCoincident);

@@ -4,7 +4,8 @@ Tolerances:
 * FP64 velocities: normwise relative error <= 1e-12 (the reference's own FP64 error against
   an exact sum is ~1e-14 normwise); per-vortex relative error (floor 1e-3 rms) <= 1e-10.
 * mixed velocities: normwise <= 1e-6 against the FP64 oracle.
-* Hamiltonians: relative 1e-11 (sums with cancellation), moments 1e-13.
+* Hamiltonians: within 16 * 2^-53 S of the extended-precision value, S the sum of the terms'
+  magnitudes (a bound from the rounding of each term and a compensated sum); moments 1e-13.
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ from oracle.pyoracle import spec
 from pointvortex_amd import (BackendRuntimeError, HipKernel, InvalidArgument, LogicError,
                              OutOfRange, SimParams, VortexSystem)
 from pointvortex_amd import workloads as wl
-from tests.conftest import fp64_gate, normwise, per_vortex
+from tests.conftest import disk_truth, fp64_gate, hamiltonian_longdouble, normwise, per_vortex
 
 pytestmark = pytest.mark.gpu
 
@@ -182,6 +183,10 @@ def test_fp64_velocity_parity(oracle, geometry, n):
     gate = fp64_gate(geometry, pop.x, pop.y, pop.circulation, ur, vr)
     assert normwise(vel.x, vel.y, ur, vr) <= gate
     assert per_vortex(vel.x, vel.y, ur, vr) <= 100 * gate
+    if geometry == "disk":
+        # Against the extended-precision sum itself, where the oracle's own FP64 loses digits.
+        ue, ve = disk_truth(pop.x, pop.y, pop.circulation)
+        assert normwise(vel.x, vel.y, ue, ve) <= 1e-12
 
 
 @pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
@@ -406,10 +411,36 @@ def test_invariants_parity(oracle, geometry, n):
                     inv.hamiltonian])
     scale = np.sum(np.abs(pop.circulation)) * max(1.0, np.max(np.abs(pop.x)) ** 2)
     assert np.all(np.abs(got[:4] - ref[:4]) <= 1e-13 * scale)
-    assert abs(got[4] - ref[4]) <= 1e-11 * max(1.0, abs(ref[4])) * max(1.0, np.sqrt(n))
+    # H against its extended-precision value: every FP64 term to a few ulps and a compensated
+    # sum leave at most a small multiple of 2^-53 S (S: the terms' magnitudes, see
+    # hamiltonian_longdouble); against the oracle, that plus the oracle's own distance.
+    h_true, s_abs = hamiltonian_longdouble(geometry, pop.x, pop.y, pop.circulation,
+                                           core_radius=kw.get("coreRadius", 0.0))
+    bar = 16 * 2.0 ** -53 * s_abs
+    assert abs(got[4] - h_true) <= bar, (got[4], h_true, bar)
+    assert abs(got[4] - ref[4]) <= abs(ref[4] - h_true) + bar
     k.uploadDeviceState(VortexSystem(pop.x, pop.y, pop.circulation))
     dev = k.deviceInvariants()
     assert dev.hamiltonian == inv.hamiltonian
+
+
+@pytest.mark.parametrize("layers", [0, 1, 2, 3, 8, 20])
+def test_periodic_hamiltonian_against_the_reference_build(reference, layers):
+    """The periodic Hamiltonian is this backend's own form (rows m != 0 as a Fourier series);
+    the reference sums the 2L+1 rows directly (src/compute.cpp:174-202).  At every L: within
+    16 * 2^-53 S of the extended-precision row sum, and against the reference build itself
+    within that plus the reference's own distance from it.  Two box lengths."""
+    for box, seed in ((2.0, 1), (TWO_PI, 2)):
+        pop = wl.periodic(400, 60 + layers + seed, box_length=box)
+        k = kernel_for("periodic", boxLengthX=box, boxLengthY=box, periodicImageLayers=layers)
+        h = k.hamiltonian(VortexSystem(pop.x, pop.y, pop.circulation))
+        sp = spec("periodic", box_length=box, image_layers=layers)
+        h_ref = reference.invariants(sp, pop.x, pop.y, pop.circulation)[4]
+        h_true, s_abs = hamiltonian_longdouble("periodic", pop.x, pop.y, pop.circulation,
+                                               box_length=box, layers=layers)
+        bar = 16 * 2.0 ** -53 * s_abs
+        assert abs(h - h_true) <= bar, (box, h, h_true, bar)
+        assert abs(h - h_ref) <= abs(h_ref - h_true) + bar, (box, h, h_ref)
 
 
 @pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
@@ -449,8 +480,8 @@ def test_fp64_parity_at_n65536_on_target_subsets(oracle, geometry):
     """The benchmark's size: the full N=65536 evaluation on the device against the oracle's
     sums for 768 targets (the first 256, 256 from the middle and the last 256, which for the
     disk are the wall shell at r in [0.995, 0.9999]) over all sources.  For the disk the bar
-    is max(1e-12, 2x the oracle's own distance from an extended-precision sum on the same
-    targets)."""
+    against the oracle is max(1e-12, 2x the oracle's own distance from an extended-precision
+    sum on the same targets), and against that sum itself 1e-12."""
     n = 65536
     if geometry == "infinite":
         pop = wl.infinite(n, 0)
@@ -470,6 +501,8 @@ def test_fp64_parity_at_n65536_on_target_subsets(oracle, geometry):
             from tests.conftest import disk_longdouble_targets
             ue, ve = disk_longdouble_targets(pop.x, pop.y, pop.circulation, np.arange(b, e))
             gate = max(gate, 2.0 * normwise(ur, vr, ue, ve))
+            # The wall shell (the last 256 targets) against the extended-precision sum.
+            assert normwise(vel.x[b:e], vel.y[b:e], ue, ve) <= 1e-12, (b, e)
         assert normwise(vel.x[b:e], vel.y[b:e], ur, vr) <= gate, (b, e)
 
 
