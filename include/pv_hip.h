@@ -281,12 +281,24 @@ typedef struct pv_slot_state {
     size_t n;
 } pv_slot_state;
 
-/* Room for `capacity` vortices and `max_instances` instances; 208 bytes of HBM per vortex. */
+/* Room for up to `capacity` vortices and `max_instances` instances.  The per-vortex arrays
+ * (208 bytes of HBM per vortex) are allocated as instances are loaded or admitted. */
 pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max_instances,
                           pv_batch **out);
 void pv_batch_destroy(pv_batch *b);
+/* The pair arithmetic of instances.  A fixed-step ensemble switches all its instances; in an
+ * ensemble of admitted runs it applies to the runs admitted from then on, each run keeping its
+ * own, so FP64 and mixed runs share one ensemble and advance in the same launches. */
 pv_status pv_batch_set_precision(pv_batch *b, int precision);
 size_t pv_batch_bytes_per_vortex(void);
+/* Admission by device footprint (no reference counterpart): with a limit of `bytes` (0: none),
+ * pv_batch_admit places a run only if everything the ensemble then holds on the device -- its
+ * per-vortex arrays, the work items and partial sums of every admitted run, and the tables --
+ * stays within it (otherwise *slot is -1 and the run can be offered again later), and
+ * pv_batch_load fails with PV_ERR_RUNTIME if the loaded ensemble would exceed it.
+ * pv_batch_device_bytes reports what the ensemble holds on the device now. */
+pv_status pv_batch_set_byte_limit(pv_batch *b, size_t bytes);
+pv_status pv_batch_device_bytes(pv_batch *b, size_t *bytes);
 
 /* How an ensemble evaluates its pairs (no reference counterpart; the reference runs one
  * system per process).  mode 1: symmetric -- work items are pairs of 128-vortex tiles of one

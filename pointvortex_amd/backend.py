@@ -168,6 +168,8 @@ _SIGNATURES = {
     "pv_batch_destroy": (None, [_vp]),
     "pv_batch_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pv_batch_set_symmetric": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pv_batch_set_byte_limit": (ctypes.c_int, [_vp, ctypes.c_size_t]),
+    "pv_batch_device_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "pv_dipole_process": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_size_t)]),
     "pv_dipole_set_random": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_char_p]),

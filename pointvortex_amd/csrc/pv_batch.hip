@@ -17,12 +17,41 @@
 #include "pv_epilogue.hpp"
 #include "pv_sym.hpp"
 
+#include <type_traits>
+
 namespace pv {
 
 namespace {
 
 constexpr int kWaveTargets = 64;
 constexpr int kBatchBlock = 256;
+
+// The pair policies of one precision, by geometry.  A launch carries one set, or two when its
+// work items mix FP64 and mixed-precision instances (each instance names its own).
+struct F64Set {
+    using I = InfF64;
+    using P = PerF64;
+    using D = DiskF64;
+};
+struct MixedSet {
+    using I = InfMixed;
+    using P = PerMixed;
+    using D = DiskMixed;
+};
+struct NoSet {};
+
+constexpr size_t max3(size_t a, size_t b, size_t c) {
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+// LDS bytes of one source record, the largest of a set's policies (0 for NoSet).
+template <class S> constexpr size_t set_src_bytes() {
+    if constexpr (std::is_same<S, NoSet>::value)
+        return 0;
+    else
+        return max3(sizeof(typename S::I::Src), sizeof(typename S::P::Src),
+                    sizeof(typename S::D::Src));
+}
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -125,14 +154,9 @@ __device__ void batch_item(const BatchLaunch &a, const Instance *ins, const Work
     }
 }
 
-template <class PI, class PP, class PD, bool ADAPT>
+template <class F, class M, bool ADAPT>
 __global__ __launch_bounds__(kBatchBlock) void batch_kernel(BatchLaunch a) {
-    constexpr size_t kSrcBytes =
-        sizeof(typename PD::Src) > sizeof(typename PI::Src)
-            ? (sizeof(typename PD::Src) > sizeof(typename PP::Src) ? sizeof(typename PD::Src)
-                                                                   : sizeof(typename PP::Src))
-            : (sizeof(typename PI::Src) > sizeof(typename PP::Src) ? sizeof(typename PI::Src)
-                                                                   : sizeof(typename PP::Src));
+    constexpr size_t kSrcBytes = max3(set_src_bytes<F>(), set_src_bytes<M>(), 0);
     __shared__ __align__(16) unsigned char lds[kBatchBlock / 64][kWaveTargets * kSrcBytes];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     unsigned char *mine = lds[wave];
@@ -152,13 +176,35 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(BatchLaunch a) {
             continue;
         if (ADAPT && (sp.state != kRunning || (a.pass == kPassNeedK0 && sp.fsal)))
             continue;
+        // The instance's policy: its geometry, and its precision when the launch carries
+        // both sets.
         const int geometry = ins->gp.geometry;
-        if (geometry == 0)
-            batch_item<PI, ADAPT>(a, ins, it, reinterpret_cast<typename PI::Src *>(mine), lane);
-        else if (geometry == 1)
-            batch_item<PP, ADAPT>(a, ins, it, reinterpret_cast<typename PP::Src *>(mine), lane);
-        else
-            batch_item<PD, ADAPT>(a, ins, it, reinterpret_cast<typename PD::Src *>(mine), lane);
+        bool done = false;
+        if constexpr (!std::is_same<M, NoSet>::value) {
+            if (ins->precision) {
+                if (geometry == 0)
+                    batch_item<typename M::I, ADAPT>(
+                        a, ins, it, reinterpret_cast<typename M::I::Src *>(mine), lane);
+                else if (geometry == 1)
+                    batch_item<typename M::P, ADAPT>(
+                        a, ins, it, reinterpret_cast<typename M::P::Src *>(mine), lane);
+                else
+                    batch_item<typename M::D, ADAPT>(
+                        a, ins, it, reinterpret_cast<typename M::D::Src *>(mine), lane);
+                done = true;
+            }
+        }
+        if (!done) {
+            if (geometry == 0)
+                batch_item<typename F::I, ADAPT>(
+                    a, ins, it, reinterpret_cast<typename F::I::Src *>(mine), lane);
+            else if (geometry == 1)
+                batch_item<typename F::P, ADAPT>(
+                    a, ins, it, reinterpret_cast<typename F::P::Src *>(mine), lane);
+            else
+                batch_item<typename F::D, ADAPT>(
+                    a, ins, it, reinterpret_cast<typename F::D::Src *>(mine), lane);
+        }
     }
 }
 
@@ -171,6 +217,15 @@ template <class P> struct SymSlice {
     typename P::Src src[kSymTile];
     double red[kSymTile][2];
 };
+
+// LDS bytes of one wave's slice, the largest of a set's policies (0 for NoSet).
+template <class S> constexpr size_t set_slice_bytes() {
+    if constexpr (std::is_same<S, NoSet>::value)
+        return 0;
+    else
+        return max3(sizeof(SymSlice<typename S::I>), sizeof(SymSlice<typename S::P>),
+                    sizeof(SymSlice<typename S::D>));
+}
 
 // One pair of tiles (I, J) of one instance: partial slot J of tile I's targets, and for I < J
 // slot I of tile J's sources (the reaction).  I == J: ordered pairs inside the tile.
@@ -329,14 +384,9 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
     }
 }
 
-template <class PI, class PP, class PD, bool ADAPT>
+template <class F, class M, bool ADAPT>
 __global__ __launch_bounds__(kBatchBlock) void batch_sym_kernel(BatchLaunch a) {
-    constexpr size_t kSliceBytes =
-        sizeof(SymSlice<PD>) > sizeof(SymSlice<PI>)
-            ? (sizeof(SymSlice<PD>) > sizeof(SymSlice<PP>) ? sizeof(SymSlice<PD>)
-                                                           : sizeof(SymSlice<PP>))
-            : (sizeof(SymSlice<PI>) > sizeof(SymSlice<PP>) ? sizeof(SymSlice<PI>)
-                                                           : sizeof(SymSlice<PP>));
+    constexpr size_t kSliceBytes = max3(set_slice_bytes<F>(), set_slice_bytes<M>(), 0);
     __shared__ __align__(16) unsigned char lds[kBatchBlock / 64][kSliceBytes];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     unsigned char *mine = lds[wave];
@@ -363,12 +413,35 @@ __global__ __launch_bounds__(kBatchBlock) void batch_sym_kernel(BatchLaunch a) {
             if (ADAPT && (sp.state != kRunning || (a.pass == kPassNeedK0 && sp.fsal)))
                 continue;
             const int geometry = ins->gp.geometry;
-            if (geometry == 0)
-                batch_sym_item<PI>(a, ins, it, reinterpret_cast<SymSlice<PI> *>(mine), lane, dir);
-            else if (geometry == 1)
-                batch_sym_item<PP>(a, ins, it, reinterpret_cast<SymSlice<PP> *>(mine), lane, dir);
-            else
-                batch_sym_item<PD>(a, ins, it, reinterpret_cast<SymSlice<PD> *>(mine), lane, dir);
+            bool done = false;
+            if constexpr (!std::is_same<M, NoSet>::value) {
+                if (ins->precision) {
+                    if (geometry == 0)
+                        batch_sym_item<typename M::I>(
+                            a, ins, it, reinterpret_cast<SymSlice<typename M::I> *>(mine), lane,
+                            dir);
+                    else if (geometry == 1)
+                        batch_sym_item<typename M::P>(
+                            a, ins, it, reinterpret_cast<SymSlice<typename M::P> *>(mine), lane,
+                            dir);
+                    else
+                        batch_sym_item<typename M::D>(
+                            a, ins, it, reinterpret_cast<SymSlice<typename M::D> *>(mine), lane,
+                            dir);
+                    done = true;
+                }
+            }
+            if (!done) {
+                if (geometry == 0)
+                    batch_sym_item<typename F::I>(
+                        a, ins, it, reinterpret_cast<SymSlice<typename F::I> *>(mine), lane, dir);
+                else if (geometry == 1)
+                    batch_sym_item<typename F::P>(
+                        a, ins, it, reinterpret_cast<SymSlice<typename F::P> *>(mine), lane, dir);
+                else
+                    batch_sym_item<typename F::D>(
+                        a, ins, it, reinterpret_cast<SymSlice<typename F::D> *>(mine), lane, dir);
+            }
         }
     }
 }
@@ -514,43 +587,40 @@ __global__ void batch_accept_kernel(const int *owner, const InstanceStep *steps,
 
 } // namespace
 
-void launch_batch(int precision, bool adaptive, const BatchLaunch &a, int blocks,
+// precisions: bit 0 set when FP64 instances are among the work items, bit 1 for mixed ones;
+// K names the kernel template (batch_kernel or batch_sym_kernel).
+#define PV_LAUNCH_SETS(K, precisions, adaptive, a, blocks, s)                                    \
+    do {                                                                                        \
+        const dim3 grid_(blocks), block_(kBatchBlock);                                         \
+        if ((precisions) == 3) {                                                                \
+            if (adaptive)                                                                       \
+                hipLaunchKernelGGL((K<F64Set, MixedSet, true>), grid_, block_, 0, s, a);        \
+            else                                                                                \
+                hipLaunchKernelGGL((K<F64Set, MixedSet, false>), grid_, block_, 0, s, a);       \
+        } else if ((precisions) == 2) {                                                         \
+            if (adaptive)                                                                       \
+                hipLaunchKernelGGL((K<MixedSet, NoSet, true>), grid_, block_, 0, s, a);         \
+            else                                                                                \
+                hipLaunchKernelGGL((K<MixedSet, NoSet, false>), grid_, block_, 0, s, a);        \
+        } else {                                                                                \
+            if (adaptive)                                                                       \
+                hipLaunchKernelGGL((K<F64Set, NoSet, true>), grid_, block_, 0, s, a);           \
+            else                                                                                \
+                hipLaunchKernelGGL((K<F64Set, NoSet, false>), grid_, block_, 0, s, a);          \
+        }                                                                                       \
+    } while (0)
+
+void launch_batch(int precisions, bool adaptive, const BatchLaunch &a, int blocks,
                   hipStream_t s) {
-    const dim3 grid(blocks), block(kBatchBlock);
-    if (precision == 0) {
-        if (adaptive)
-            hipLaunchKernelGGL((batch_kernel<InfF64, PerF64, DiskF64, true>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((batch_kernel<InfF64, PerF64, DiskF64, false>), grid, block, 0, s, a);
-    } else {
-        if (adaptive)
-            hipLaunchKernelGGL((batch_kernel<InfMixed, PerMixed, DiskMixed, true>), grid, block, 0,
-                               s, a);
-        else
-            hipLaunchKernelGGL((batch_kernel<InfMixed, PerMixed, DiskMixed, false>), grid, block, 0,
-                               s, a);
-    }
+    PV_LAUNCH_SETS(batch_kernel, precisions, adaptive, a, blocks, s);
 }
 
-void launch_batch_sym(int precision, bool adaptive, const BatchLaunch &a, int blocks,
+void launch_batch_sym(int precisions, bool adaptive, const BatchLaunch &a, int blocks,
                       hipStream_t s) {
-    const dim3 grid(blocks), block(kBatchBlock);
-    if (precision == 0) {
-        if (adaptive)
-            hipLaunchKernelGGL((batch_sym_kernel<InfF64, PerF64, DiskF64, true>), grid, block, 0,
-                               s, a);
-        else
-            hipLaunchKernelGGL((batch_sym_kernel<InfF64, PerF64, DiskF64, false>), grid, block, 0,
-                               s, a);
-    } else {
-        if (adaptive)
-            hipLaunchKernelGGL((batch_sym_kernel<InfMixed, PerMixed, DiskMixed, true>), grid,
-                               block, 0, s, a);
-        else
-            hipLaunchKernelGGL((batch_sym_kernel<InfMixed, PerMixed, DiskMixed, false>), grid,
-                               block, 0, s, a);
-    }
+    PV_LAUNCH_SETS(batch_sym_kernel, precisions, adaptive, a, blocks, s);
 }
+
+#undef PV_LAUNCH_SETS
 
 void launch_batch_sym_reduce(bool adaptive, const BatchLaunch &a, hipStream_t s) {
     const dim3 grid((unsigned)((a.span + kBatchBlock - 1) / kBatchBlock)), block(kBatchBlock);

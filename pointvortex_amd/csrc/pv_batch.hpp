@@ -12,7 +12,7 @@ struct Instance {
     int offset;
     int n;
     int nt;
-    int pad;
+    int precision;   // the instance's pair arithmetic: 0 FP64, 1 mixed
     size_t poff;
 };
 
@@ -94,12 +94,13 @@ struct BatchLaunch {
     int span;                  // flat vortices the reduction covers
 };
 
-// Launchers (pv_batch.hip).
-void launch_batch(int precision, bool adaptive, const BatchLaunch &a, int blocks, hipStream_t s);
+// Launchers (pv_batch.hip).  `precisions` says which pair arithmetic the work items' instances
+// use: bit 0 FP64, bit 1 mixed (both: one launch dispatches on each instance's precision).
+void launch_batch(int precisions, bool adaptive, const BatchLaunch &a, int blocks, hipStream_t s);
 void launch_batch_reset(unsigned *counter, hipStream_t s);
 // Symmetric mode: the pair kernel over PairItems, then the per-vortex reduction with the
 // stage epilogue.
-void launch_batch_sym(int precision, bool adaptive, const BatchLaunch &a, int blocks,
+void launch_batch_sym(int precisions, bool adaptive, const BatchLaunch &a, int blocks,
                       hipStream_t s);
 void launch_batch_sym_reduce(bool adaptive, const BatchLaunch &a, hipStream_t s);
 void launch_batch_step_done(unsigned *counter, hipStream_t s);

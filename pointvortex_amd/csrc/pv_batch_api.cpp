@@ -85,10 +85,16 @@ void validate_instance(const pv_instance_config &c, const double *x, const doubl
 
 struct pv_batch {
     int device = 0;
-    int precision = 0;
+    int precision = 0;                 // of the instances loaded or admitted next
     int cu_count = 256;
     hipStream_t stream = nullptr;
     size_t capacity = 0, max_inst = 0;
+    // The per-vortex arrays below hold vcap <= capacity vortices; they grow as instances are
+    // loaded or admitted.  byte_limit (0: none) bounds everything the ensemble allocates on
+    // the device: an admission that would exceed it is refused.
+    size_t vcap = 0;
+    size_t byte_limit = 0;
+    int item_precisions = 1;           // bit 0: FP64 instances among the work items, bit 1 mixed
 
     double *x = nullptr, *y = nullptr, *g = nullptr, *xn = nullptr, *yn = nullptr;
     double *xa = nullptr, *ya = nullptr, *xb = nullptr, *yb = nullptr;
@@ -195,13 +201,13 @@ void launch(pv_batch *b, const double *px, const double *py, const pv::StageOp &
         a.part = b->part;
         a.owner = b->owner;
         a.span = b->span;
-        pv::launch_batch_sym(b->precision, adaptive, a, batch_blocks(b), b->stream);
+        pv::launch_batch_sym(b->item_precisions, adaptive, a, batch_blocks(b), b->stream);
         hip_check(hipGetLastError(), "launch HIP ensemble pair kernel");
         pv::launch_batch_sym_reduce(adaptive, a, b->stream);
         hip_check(hipGetLastError(), "launch HIP ensemble reduction");
         return;
     }
-    pv::launch_batch(b->precision, adaptive, a, batch_blocks(b), b->stream);
+    pv::launch_batch(b->item_precisions, adaptive, a, batch_blocks(b), b->stream);
     hip_check(hipGetLastError(), "launch HIP ensemble kernel");
 }
 
@@ -225,7 +231,7 @@ void build_pair_items(pv_batch *b, const std::vector<int> &instances) {
         in.nt = nt;
         in.poff = poff;
         poff += (size_t)nt * 2 * (size_t)in.n;
-        const double w = pair_weight(in.gp.geometry, b->precision);
+        const double w = pair_weight(in.gp.geometry, in.precision);
         for (int J = 0; J < nt; ++J) {
             const double rj = std::min(TT, in.n - J * TT);
             for (int I = 0; I <= J; ++I) {
@@ -305,7 +311,7 @@ void build_ordered_items(pv_batch *b, const std::vector<int> &instances) {
     std::vector<Keyed> v;
     for (int k : instances) {
         const int n = b->inst_h[k].n;
-        const double w = pair_weight(b->inst_h[k].gp.geometry, b->precision) * n;
+        const double w = pair_weight(b->inst_h[k].gp.geometry, b->inst_h[k].precision) * n;
         for (int tb = 0; tb < n; tb += 64)
             v.push_back({w, {k, tb, std::min(n, tb + 64), 0}});
     }
@@ -333,6 +339,10 @@ void build_ordered_items(pv_batch *b, const std::vector<int> &instances) {
 void build_items(pv_batch *b, const std::vector<int> &instances) {
     b->item_instances = instances;
     b->sym = wants_sym(b);
+    int mask = 0;
+    for (int k : instances)
+        mask |= b->inst_h[k].precision ? 2 : 1;
+    b->item_precisions = mask ? mask : 1;
     if (b->sym)
         build_pair_items(b, instances);
     else
@@ -395,11 +405,13 @@ void clear_instances(pv_batch *b) {
     hip_check(hipMemcpyAsync(b->steps_d, b->steps_h, b->max_inst * sizeof(pv::InstanceStep),
                              hipMemcpyHostToDevice, b->stream),
               "upload instance states");
-    std::vector<int> minus(b->capacity, -1);
-    hip_check(hipMemcpyAsync(b->owner, minus.data(), b->capacity * sizeof(int),
-                             hipMemcpyHostToDevice, b->stream),
-              "clear owners");
-    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    if (b->vcap) {
+        std::vector<int> minus(b->vcap, -1);
+        hip_check(hipMemcpyAsync(b->owner, minus.data(), b->vcap * sizeof(int),
+                                 hipMemcpyHostToDevice, b->stream),
+                  "clear owners");
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    }
     b->used.assign(b->max_inst, 0);
     b->in_items.assign(b->max_inst, 0);
     b->free_ranges = {{0, b->capacity}};
@@ -459,11 +471,108 @@ void require_fixed(pv_batch *b) {
         raise(PV_ERR_LOGIC, "ensemble holds admitted runs; fixed-step calls need pv_batch_load");
 }
 
+// The 25 per-vortex FP64 arrays.
+std::vector<double **> vortex_arrays(pv_batch *b) {
+    std::vector<double **> v = {&b->x, &b->y, &b->g, &b->xn, &b->yn, &b->xa, &b->ya, &b->xb,
+                                &b->yb, &b->u, &b->v};
+    for (int s = 0; s < 7; ++s) {
+        v.push_back(&b->kx[s]);
+        v.push_back(&b->ky[s]);
+    }
+    return v;
+}
+
+constexpr size_t kVortexBytes = kDoublesPerVortex * sizeof(double) + sizeof(int);
+
+// Device bytes that do not depend on the vortices: instance tables, counters, status.
+size_t fixed_bytes(const pv_batch *b) {
+    return b->max_inst * (sizeof(pv::Instance) + sizeof(pv::InstanceStep)) +
+           2 * sizeof(unsigned) + sizeof(pv::DevStatus);
+}
+
+// Everything the ensemble holds on the device now.
+size_t device_bytes(const pv_batch *b) {
+    return fixed_bytes(b) + b->vcap * kVortexBytes + b->part_cap * sizeof(double) +
+           b->pitems_cap * (sizeof(pv::PairItem) + sizeof(int)) +
+           (b->pitems_cap ? sizeof(int) : 0) + b->items_cap * sizeof(pv::WorkItem);
+}
+
+// Work-item and partial-sum bytes one running instance of n vortices needs in `sym` mode.
+size_t instance_bytes(size_t n, bool sym) {
+    if (!sym)
+        return (n + 63) / 64 * sizeof(pv::WorkItem);
+    const size_t nt = (n + pv::kSymTile - 1) / pv::kSymTile;
+    return nt * (nt + 1) / 2 * (sizeof(pv::PairItem) + sizeof(int)) + sizeof(int) +
+           nt * 2 * n * sizeof(double);
+}
+
+// The per-vortex arrays grown to newcap vortices (contents kept, new owners -1), one array at
+// a time so that at most one old array is held beside the new ones.  Captured graphs hold the
+// old pointers and are dropped.
+void grow_vortices(pv_batch *b, size_t newcap) {
+    if (newcap <= b->vcap)
+        return;
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    for (auto &kv : b->graphs)
+        (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    const size_t old = b->vcap;
+    for (double **p : vortex_arrays(b)) {
+        double *q = dalloc<double>(newcap, "hipMalloc(ensemble state)");
+        if (*p && old)
+            hip_check(hipMemcpyAsync(q, *p, old * sizeof(double), hipMemcpyDeviceToDevice,
+                                     b->stream),
+                      "copy ensemble state");
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        dfree(*p);
+        *p = q;
+    }
+    int *o = dalloc<int>(newcap, "hipMalloc(owners)");
+    if (b->owner && old)
+        hip_check(hipMemcpyAsync(o, b->owner, old * sizeof(int), hipMemcpyDeviceToDevice,
+                                 b->stream),
+                  "copy owners");
+    std::vector<int> minus(newcap - old, -1);
+    hip_check(hipMemcpyAsync(o + old, minus.data(), minus.size() * sizeof(int),
+                             hipMemcpyHostToDevice, b->stream),
+              "clear owners");
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    dfree(b->owner);
+    b->owner = o;
+    b->vcap = newcap;
+}
+
+// Frees the work-item and partial-sum buffers; the next attempt rebuilds them at the size the
+// running instances need.
+void trim_items(pv_batch *b) {
+    hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+    dfree(b->part), dfree(b->pitems_d), dfree(b->chunks_d), dfree(b->items_d);
+    b->part_cap = b->pitems_cap = b->items_cap = 0;
+    b->nitems = b->nchunks = 0;
+    b->items_dirty = true;
+}
+
 } // namespace
 
 extern "C" {
 
 size_t pv_batch_bytes_per_vortex(void) { return kDoublesPerVortex * sizeof(double) + 8; }
+
+pv_status pv_batch_set_byte_limit(pv_batch *b, size_t bytes) {
+    return guard([&] {
+        checked(b);
+        b->byte_limit = bytes;
+    });
+}
+
+pv_status pv_batch_device_bytes(pv_batch *b, size_t *bytes) {
+    return guard([&] {
+        checked(b);
+        if (!bytes)
+            raise(PV_ERR_INVALID_ARGUMENT, "null output");
+        *bytes = device_bytes(b);
+    });
+}
 
 size_t pv_batch_symmetric_bytes(size_t n) {
     const size_t tiles = (n + pv::kSymTile - 1) / pv::kSymTile;
@@ -495,23 +604,6 @@ pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max
         hip_check(hipGetDeviceProperties(&prop, device), "query HIP device properties");
         b->cu_count = std::max(1, prop.multiProcessorCount);
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "create HIP stream");
-        const size_t n = capacity;
-        b->x = dalloc<double>(n, "hipMalloc(x)");
-        b->y = dalloc<double>(n, "hipMalloc(y)");
-        b->g = dalloc<double>(n, "hipMalloc(circulation)");
-        b->xn = dalloc<double>(n, "hipMalloc(next x)");
-        b->yn = dalloc<double>(n, "hipMalloc(next y)");
-        b->xa = dalloc<double>(n, "hipMalloc(stage x)");
-        b->ya = dalloc<double>(n, "hipMalloc(stage y)");
-        b->xb = dalloc<double>(n, "hipMalloc(stage x)");
-        b->yb = dalloc<double>(n, "hipMalloc(stage y)");
-        for (int s = 0; s < 7; ++s) {
-            b->kx[s] = dalloc<double>(n, "hipMalloc(x stage)");
-            b->ky[s] = dalloc<double>(n, "hipMalloc(y stage)");
-        }
-        b->u = dalloc<double>(n, "hipMalloc(u)");
-        b->v = dalloc<double>(n, "hipMalloc(v)");
-        b->owner = dalloc<int>(n, "hipMalloc(owners)");
         b->inst_d = dalloc<pv::Instance>(max_instances, "hipMalloc(instances)");
         b->steps_d = dalloc<pv::InstanceStep>(max_instances, "hipMalloc(instance states)");
         hip_check(hipHostMalloc((void **)&b->steps_h, max_instances * sizeof(pv::InstanceStep),
@@ -526,10 +618,6 @@ pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max
         hip_check(hipMemcpyAsync(b->steps_d, b->steps_h, max_instances * sizeof(pv::InstanceStep),
                                  hipMemcpyHostToDevice, b->stream),
                   "upload instance states");
-        std::vector<int> minus(n, -1);
-        hip_check(hipMemcpyAsync(b->owner, minus.data(), n * sizeof(int), hipMemcpyHostToDevice,
-                                 b->stream),
-                  "clear owners");
         hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
         b->counter = dalloc<unsigned>(2, "hipMalloc(work counter)");
         hip_check(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), b->stream),
@@ -563,10 +651,17 @@ pv_status pv_batch_set_precision(pv_batch *b, int precision) {
         if (precision != PV_FP64 && precision != PV_MIXED)
             raise(PV_ERR_INVALID_ARGUMENT, "unsupported precision");
         b->precision = precision;
+        if (b->adaptive)
+            return;     // admitted runs keep theirs; the next admissions take this one
+        DeviceGuard dg(b->device);
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
         for (auto &kv : b->graphs)
             (void)hipGraphExecDestroy(kv.second);
         b->graphs.clear();
-        b->items_dirty = true;
+        for (size_t k = 0; k < b->nfixed; ++k)
+            b->inst_h[k].precision = precision;
+        if (!b->item_instances.empty())
+            build_items(b, std::vector<int>(b->item_instances));
     });
 }
 
@@ -603,8 +698,16 @@ pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const si
             total += counts[k];
         if (total > b->capacity)
             raise(PV_ERR_INVALID_ARGUMENT, "more vortices than the ensemble was created for");
+        if (b->byte_limit) {
+            size_t need = fixed_bytes(b) + std::max(total, b->vcap) * kVortexBytes;
+            for (size_t k = 0; k < ninst; ++k)
+                need += instance_bytes(counts[k], b->sym_mode != 0);
+            if (need > b->byte_limit)
+                raise(PV_ERR_RUNTIME, "ensemble would exceed its device byte limit");
+        }
         DeviceGuard dg(b->device);
         clear_instances(b);
+        grow_vortices(b, total);
         size_t off = 0;
         std::vector<int> owner(total);
         std::vector<int> all;
@@ -615,6 +718,7 @@ pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const si
             b->inst_h[k].gp = geo_params(c);
             b->inst_h[k].offset = (int)off;
             b->inst_h[k].n = (int)counts[k];
+            b->inst_h[k].precision = b->precision;
             b->steps_h[k].state = pv::kRunning;
             std::fill(owner.begin() + off, owner.begin() + off + counts[k], (int)k);
             if (counts[k])
@@ -811,10 +915,38 @@ pv_status pv_batch_admit(pv_batch *b, const pv_instance_config *config, const do
         if (r == b->free_ranges.size())
             return;
         const size_t off = b->free_ranges[r].first;
+        // Admission by device footprint: the per-vortex arrays grown to cover the run, the work
+        // items and partial sums every admitted run needs, and (while the arrays grow) one old
+        // array held beside the new ones, all within the byte limit.
+        DeviceGuard dg(b->device);
+        const size_t need = off + n;
+        size_t newcap = b->vcap;
+        if (need > b->vcap)
+            newcap = std::min(b->capacity,
+                              std::max({need, b->vcap + b->vcap / 2, (size_t)1 << 16}));
+        if (b->byte_limit) {
+            const bool sym = b->sym_mode == 1;
+            size_t items = instance_bytes(n, sym);
+            for (size_t q = 0; q < b->max_inst; ++q)
+                if (b->used[q])
+                    items += instance_bytes((size_t)b->inst_h[q].n, sym);
+            const size_t held = device_bytes(b) - fixed_bytes(b) - b->vcap * kVortexBytes;
+            auto projected = [&](size_t cap, size_t buffers) {
+                const size_t growing = cap > b->vcap ? b->vcap * sizeof(double) : 0;
+                return fixed_bytes(b) + cap * kVortexBytes + growing + buffers;
+            };
+            if (projected(newcap, std::max(items, held)) > b->byte_limit)
+                newcap = std::max(need, b->vcap);
+            if (projected(newcap, std::max(items, held)) > b->byte_limit) {
+                if (projected(newcap, items) > b->byte_limit)
+                    return;                 // does not fit now
+                trim_items(b);              // buffers still sized for runs that have left
+            }
+        }
+        grow_vortices(b, newcap);
         b->free_ranges[r].first += n;
         if (b->free_ranges[r].first == b->free_ranges[r].second)
             b->free_ranges.erase(b->free_ranges.begin() + r);
-        DeviceGuard dg(b->device);
         const size_t bytes = n * sizeof(double);
         hip_check(hipMemcpyAsync(b->x + off, x, bytes, hipMemcpyHostToDevice, b->stream), "copy x");
         hip_check(hipMemcpyAsync(b->y + off, y, bytes, hipMemcpyHostToDevice, b->stream), "copy y");
@@ -823,6 +955,7 @@ pv_status pv_batch_admit(pv_batch *b, const pv_instance_config *config, const do
         b->inst_h[k].gp = geo_params(c);
         b->inst_h[k].offset = (int)off;
         b->inst_h[k].n = (int)n;
+        b->inst_h[k].precision = b->precision;
         pv::InstanceStep &s = b->steps_h[k];
         std::memset(&s, 0, sizeof s);
         // src/main.cpp:162-165 (initial dt) and 312-314 (landing on the end time).

@@ -83,7 +83,19 @@ class Ensemble:
             pass
 
     def set_precision(self, precision: str):
+        """Fixed-step ensembles: every instance.  Admitted runs: the runs admitted next (each
+        run keeps its own; FP64 and mixed runs advance in the same launches)."""
         _check(self.lib.pv_batch_set_precision(self._h, PRECISIONS[precision]))
+
+    def set_byte_limit(self, nbytes: int):
+        """Admit runs only while everything the ensemble holds on the device stays within
+        `nbytes` (0: no limit)."""
+        _check(self.lib.pv_batch_set_byte_limit(self._h, int(nbytes)))
+
+    def device_bytes(self) -> int:
+        out = ctypes.c_size_t(0)
+        _check(self.lib.pv_batch_device_bytes(self._h, ctypes.byref(out)))
+        return out.value
 
     def set_symmetric(self, mode: int):
         """1: each unordered pair once over tile pairs; 0: ordered 64-target items; -1
@@ -145,8 +157,11 @@ class Ensemble:
     # -- adaptive runs
     def admit(self, config: InstanceConfig, x, y, g, t_end: float, dt0: float = 1e-3,
               atol: float = 1e-10, rtol: float = 1e-8, min_dt: float = 1e-12,
-              max_dt: float = 0.1) -> Optional[int]:
-        """Places one DOPRI5 run; returns its slot, or None when it does not fit now."""
+              max_dt: float = 0.1, precision: Optional[str] = None) -> Optional[int]:
+        """Places one DOPRI5 run (in `precision`, default the ensemble's current one); returns
+        its slot, or None when it does not fit now."""
+        if precision is not None:
+            self.set_precision(precision)
         x, y, g = _f64(x), _f64(y), _f64(g)
         slot = ctypes.c_int(-1)
         c = config.c()

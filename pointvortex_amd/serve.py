@@ -1,22 +1,24 @@
-"""A request loop with continuous batching over adaptive ensembles.
+"""A request loop with continuous batching over an adaptive ensemble.
 
 Requests (geometry, N, circulations, horizon, tolerance) arrive over time.  Between DOPRI5
-attempts the loop admits, in arrival order, every arrived request whose HBM footprint fits the
-budget -- 208 bytes per vortex of ensemble state, plus the partial sums of the symmetric
-decomposition (16 bytes per vortex per 128 vortices of the run) -- so new runs join runs
-already in flight; finished or failed runs leave and free their bytes.  Each run keeps its
-state, FSAL derivative and step-size controller on the device; one attempt advances every
-running run by one DOPRI5 attempt (seven stage evaluations).  Between polls the loop runs up
-to `max_batch` attempts back to back with one synchronisation: one while an arrived request
-waits for budget, never past the next arrival, and never more than a quarter of the attempts
-the youngest running run has taken, so that a run finishes at most about a quarter of its
-attempts before the loop sees it.  Runs are split into pairs of
-128-vortex tiles, each unordered pair computed once (`Ensemble.set_symmetric`), so a lone
-large run still spreads over the GPU.  Latency is measured from arrival to completion.
+attempts the loop admits, in arrival order, every arrived request that fits the HBM budget, so
+new runs join runs already in flight; finished or failed runs leave and free their bytes.  The
+budget is the ensemble's device byte limit: a run is admitted only if everything the ensemble
+then holds on the device -- its per-vortex state (208 bytes per vortex), the work items and
+partial sums of every admitted run (about 16 bytes per vortex per 128 vortices of the run), and
+its tables -- stays within it (`Ensemble.set_byte_limit`).  Each run keeps its state, FSAL
+derivative and step-size controller on the device; one attempt advances every running run by
+one DOPRI5 attempt (seven stage evaluations).  Between polls the loop runs up to `max_batch`
+attempts back to back with one synchronisation: one while an arrived request waits for budget,
+never past the next arrival, and never more than a quarter of the attempts the youngest running
+run has taken, so that a run finishes at most about a quarter of its attempts before the loop
+sees it.  Runs are split into pairs of 128-vortex tiles, each unordered pair computed once
+(`Ensemble.set_symmetric`), so a lone large run still spreads over the GPU.  Latency is
+measured from arrival to completion.
 
 Precision policy: a request runs in mixed precision when its rtol is at least
-`mixed_rtol_threshold` (1e-6 by default), otherwise in FP64; each precision has its own
-ensemble.
+`mixed_rtol_threshold` (1e-6 by default), otherwise in FP64.  Both kinds of run live in the
+one ensemble, each with its own arithmetic, and advance in the same persistent launches.
 """
 from __future__ import annotations
 
@@ -98,38 +100,46 @@ class Server:
         self.per_vortex = bytes_per_vortex()
         self.budget = int(hbm_budget_bytes)
         self.capacity = max(1, self.budget // self.per_vortex)
-        self.used = 0                      # footprint of the admitted runs, both precisions
+        self.used = 0                      # footprint of the admitted runs (estimate)
         self.peak_used = 0
+        self.peak_device_bytes = 0         # what the ensemble held on the device, at most
+        self.mixed_and_fp64_at_once = False   # runs of both precisions were admitted together
         self.max_runs = max_runs
         self.device = device
         self.threshold = mixed_rtol_threshold
         self.keep = keep_results
         self.symmetric = symmetric
         self.max_batch = max(1, int(max_batch))
-        self.engines: Dict[str, Ensemble] = {}
+        self._engine: Optional[Ensemble] = None
 
     def footprint(self, n: int) -> int:
-        """HBM bytes a run of n vortices holds while admitted."""
+        """HBM bytes a run of n vortices holds while admitted (state and partial sums)."""
         return n * self.per_vortex + (symmetric_bytes(n) if self.symmetric else 0)
 
     def precision_for(self, req: Request) -> str:
         return "mixed" if req.rtol >= self.threshold else "fp64"
 
-    def engine(self, precision: str) -> Ensemble:
-        if precision not in self.engines:
-            e = Ensemble(self.capacity, self.max_runs, precision, self.device)
+    @property
+    def engine(self) -> Ensemble:
+        if self._engine is None:
+            e = Ensemble(self.capacity, self.max_runs, "fp64", self.device)
             e.set_symmetric(1 if self.symmetric else 0)
-            self.engines[precision] = e
-        return self.engines[precision]
+            e.set_byte_limit(self.budget)
+            self._engine = e
+        return self._engine
+
+    def _track(self):
+        self.peak_device_bytes = max(self.peak_device_bytes, self.engine.device_bytes())
 
     def run(self, requests: List[Request], clock=time.perf_counter,
             max_wall: float = float("inf"), on_result=None) -> ServerStats:
         """Serves `requests` (arrival times relative to the start) until all are done or
         `max_wall` seconds have passed; `on_result(result, stats)` is called as each run
         leaves."""
+        eng = self.engine
         pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
-        active: Dict[tuple, tuple] = {}
-        tried: Dict[tuple, int] = {}       # attempts each running run has taken so far
+        active: Dict[int, tuple] = {}      # slot -> (request, admitted time, precision)
+        tried: Dict[int, int] = {}         # attempts each running run has taken so far
         per_attempt = 0.0                  # running average of the wall time of one attempt
         stats = ServerStats()
         t0 = clock()
@@ -141,20 +151,20 @@ class Server:
             while pending and pending[0].arrival <= now:
                 req = pending[0]
                 prec = self.precision_for(req)
-                need = self.footprint(req.n)
-                slot = None
-                if self.used + need <= self.budget:
-                    pop, cfg = req.population()
-                    slot = self.engine(prec).admit(cfg, pop.x, pop.y, pop.circulation,
-                                                   req.t_end, req.dt0, req.atol, req.rtol)
+                pop, cfg = req.population()
+                slot = eng.admit(cfg, pop.x, pop.y, pop.circulation, req.t_end, req.dt0,
+                                 req.atol, req.rtol, precision=prec)
                 if slot is None:
                     if not active:
                         raise RuntimeError(f"request {req.rid} (N={req.n}) exceeds the budget")
                     break
-                self.used += need
+                self._track()
+                self.used += self.footprint(req.n)
                 self.peak_used = max(self.peak_used, self.used)
-                active[(prec, slot)] = (req, clock() - t0)
+                active[slot] = (req, clock() - t0, prec)
                 pending.popleft()
+                if len({p for _, _, p in active.values()}) == 2:
+                    self.mixed_and_fp64_at_once = True
             if not active:
                 wait = pending[0].arrival - (clock() - t0)
                 if wait > 0:
@@ -167,32 +177,29 @@ class Server:
                     batch = min(batch, int((pending[0].arrival - now) / per_attempt))
                 batch = max(1, batch)
             started = clock()
-            for prec, eng in self.engines.items():
-                if any(k[0] == prec for k in active):
-                    eng.attempt(batch)
-                    stats.attempts += batch
+            eng.attempt(batch)
+            stats.attempts += batch
+            self._track()
             elapsed = (clock() - started) / batch
             per_attempt = elapsed if per_attempt == 0.0 else 0.8 * per_attempt + 0.2 * elapsed
             now = clock() - t0
-            for prec, eng in self.engines.items():
-                for s in eng.poll():
-                    key = (prec, s.slot)
-                    if key not in active:
-                        continue
-                    if s.state == "running":
-                        tried[key] = s.accepted + s.rejected
-                        continue
-                    tried.pop(key, None)
-                    req, admitted = active.pop(key)
-                    x = y = None
-                    if self.keep and s.state == "finished":
-                        x, y = eng.fetch(s.slot, s.n)
-                    eng.release(s.slot)
-                    self.used -= self.footprint(req.n)
-                    stats.results.append(Result(req.rid, s.state, req.arrival, admitted, now,
-                                                s.accepted, s.rejected, s.failure, prec, x, y))
-                    if on_result is not None:
-                        on_result(stats.results[-1], stats)
+            for s in eng.poll():
+                if s.slot not in active:
+                    continue
+                if s.state == "running":
+                    tried[s.slot] = s.accepted + s.rejected
+                    continue
+                tried.pop(s.slot, None)
+                req, admitted, prec = active.pop(s.slot)
+                x = y = None
+                if self.keep and s.state == "finished":
+                    x, y = eng.fetch(s.slot, s.n)
+                eng.release(s.slot)
+                self.used -= self.footprint(req.n)
+                stats.results.append(Result(req.rid, s.state, req.arrival, admitted, now,
+                                            s.accepted, s.rejected, s.failure, prec, x, y))
+                if on_result is not None:
+                    on_result(stats.results[-1], stats)
         stats.wall = clock() - t0
         return stats
 

@@ -188,3 +188,56 @@ def test_c4_failing_disk_instances_fail_in_the_oracle_too(oracle, c4):
             s_twin, m_twin = oracle_failing_step(oracle, c, x, y, p.circulation, 1e-3, 500)
             window.append(s_twin if m_twin else 500)
         assert min(window) <= done <= max(window), (k, p.n, done, window)
+
+
+# ------------------------------------------------------------------ serving with large runs
+
+
+@pytest.mark.timeout(1200)
+def test_c5_serving_admits_large_runs_under_a_binding_budget(oracle):
+    """The serving loop with runs at N=131072 and N=65536 (disk populations of the benchmark's
+    kind, FP64 at rtol 1e-8) beside small runs of both precisions, under a 2.5 GB budget that
+    cannot hold both large runs at once: the second waits until the first has left.  The
+    device allocation never exceeds the budget; each large run gives the same bits as when
+    served alone; and the final state's FP64 velocities agree with the oracle on the first 256
+    targets and the wall shell (the last 256)."""
+    from pointvortex_amd.serve import Request, Server
+
+    # From dt = 1e-6 the controller needs at least six accepted steps (at most 5x growth per
+    # step) to reach t = 1e-3, more where the closest pairs limit the step.
+    big = [Request(0, 0.0, "disk", 131072, 71, 1e-3, 1e-8, 1e-11, dt0=1e-6),
+           Request(1, 0.0, "disk", 65536, 72, 1e-3, 1e-8, 1e-11, dt0=1e-6)]
+    small = [Request(2 + k, 0.0, ["infinite", "periodic", "disk"][k % 3], 64 + 96 * k,
+                     80 + k, 2e-4, 1e-8 if k % 2 else 1e-5, 1e-11 if k % 2 else 1e-8)
+             for k in range(10)]
+    reqs = big + small
+    budget = int(2.5e9)
+    srv = Server(hbm_budget_bytes=budget, max_runs=16, keep_results=True)
+    assert srv.footprint(131072) + srv.footprint(65536) > budget
+    stats = srv.run(reqs, max_wall=600.0)
+    res = {r.rid: r for r in stats.results}
+    assert len(res) == len(reqs) and all(r.state == "finished" for r in res.values()), \
+        [(r.rid, r.state, r.failure) for r in stats.results]
+    assert 0 < srv.peak_device_bytes <= budget
+    assert res[1].admitted >= res[0].finished          # the budget bound
+    assert {r.precision for r in res.values()} == {"fp64", "mixed"}
+    for req in big:
+        r = res[req.rid]
+        assert 6 <= r.accepted + r.rejected <= 600, (req.n, r.accepted, r.rejected)
+        alone = Server(hbm_budget_bytes=budget, max_runs=2, keep_results=True)
+        a = alone.run([req], max_wall=600.0).results[0]
+        assert a.state == "finished"
+        assert (a.accepted, a.rejected) == (r.accepted, r.rejected)
+        assert np.array_equal(a.x, r.x) and np.array_equal(a.y, r.y), req.n
+        pop, _ = req.population()
+        assert np.all(r.x * r.x + r.y * r.y < 1.0)
+        k = HipKernel(SimParams(boundaryCondition="disk"))
+        vel = k.evaluate(VortexSystem(r.x, r.y, pop.circulation))
+        n = req.n
+        for b, e in [(0, 256), (n - 256, n)]:
+            ur, vr = oracle.evaluate_range(spec("disk"), r.x, r.y, pop.circulation, b, e)
+            from tests.conftest import disk_longdouble_targets
+            ue, ve = disk_longdouble_targets(r.x, r.y, pop.circulation, np.arange(b, e))
+            assert normwise(vel.x[b:e], vel.y[b:e], ue, ve) <= 1e-12, (n, b)
+            gate = max(1e-12, 2.0 * normwise(ur[b:e], vr[b:e], ue, ve))
+            assert normwise(vel.x[b:e], vel.y[b:e], ur[b:e], vr[b:e]) <= gate, (n, b)

@@ -485,3 +485,58 @@ def test_serving_budget_counts_both_precisions_and_partials():
     assert srv.peak_used <= srv.budget
     assert srv.peak_used >= 2 * min(srv.footprint(r.n) for r in reqs)   # runs overlapped
     assert srv.used == 0
+    # What the one ensemble actually held on the device, both precisions in it at once.
+    assert {r.precision for r in stats.results} == {"fp64", "mixed"}
+    assert srv.mixed_and_fp64_at_once
+    assert 0 < srv.peak_device_bytes <= srv.budget
+
+
+def test_mixed_and_fp64_runs_in_one_ensemble_equal_their_runs_alone():
+    """FP64 and mixed runs admitted into one ensemble advance in the same launches (one kernel
+    dispatching on each run's precision), and each gives the same bits as alone in an
+    ensemble of its own precision."""
+    cfgs, pops = mixed_population(6, 31, n_min=40, n_max=300)
+    precs = ["fp64", "mixed"] * 3
+    kw = dict(t_end=0.002, dt0=1e-4, atol=1e-10, rtol=1e-8)
+    both = Ensemble(4096, 8)
+    both.set_symmetric(1)
+    slots = [both.admit(c, p.x, p.y, p.circulation, precision=q, **kw)
+             for c, p, q in zip(cfgs, pops, precs)]
+    drive(both, 3000)
+    st = {s.slot: s for s in both.poll()}
+    for c, p, q, k in zip(cfgs, pops, precs, slots):
+        alone = Ensemble(1024, 2, q)
+        alone.set_symmetric(1)
+        a = alone.admit(c, p.x, p.y, p.circulation, **kw)
+        drive(alone, 3000)
+        sa = alone.poll()[0]
+        assert sa.state == st[k].state == "finished"
+        assert (sa.accepted, sa.rejected) == (st[k].accepted, st[k].rejected)
+        xa, ya = alone.fetch(a, p.n)
+        xb, yb = both.fetch(k, p.n)
+        assert np.array_equal(xa, xb) and np.array_equal(ya, yb), (c, p.n, q)
+
+
+def test_byte_limit_bounds_what_the_ensemble_holds():
+    """With a byte limit, admission refuses a run whose state, items and partial sums would
+    take the ensemble's device allocation past it; after the others leave it fits.  (Runs of
+    about 20000 vortices, whose partial sums outweigh their state.)"""
+    e = Ensemble(100_000, 8)
+    e.set_symmetric(1)
+    p, q = wl.infinite(20000, 1), wl.infinite(18000, 2)
+    e.set_byte_limit(0)
+    a = e.admit(InstanceConfig(), p.x, p.y, p.circulation, t_end=1e-4)
+    e.attempt()
+    one = e.device_bytes()
+    e.release(a)
+    e.set_byte_limit(int(1.5 * one))
+    a = e.admit(InstanceConfig(), p.x, p.y, p.circulation, t_end=1e-4)
+    assert a is not None
+    assert e.admit(InstanceConfig(), q.x, q.y, q.circulation, t_end=1e-4) is None
+    e.attempt()
+    assert e.device_bytes() <= int(1.5 * one)
+    e.release(a)
+    b = e.admit(InstanceConfig(), q.x, q.y, q.circulation, t_end=1e-4)
+    assert b is not None
+    e.attempt()
+    assert e.device_bytes() <= int(1.5 * one)

@@ -95,7 +95,8 @@ def _loop_of_each_line(lines):
 
 def test_every_persistent_kernel_was_found(batch_asm):
     names = [n for n, _ in _kernels(batch_asm)]
-    assert len(names) == 8, names      # {ordered, symmetric} x {fp64, mixed} x {fixed, adaptive}
+    # {ordered, symmetric} x {fp64, mixed, both in one launch} x {fixed, adaptive}
+    assert len(names) == 12, names
 
 
 def test_item_fetch_and_item_load_share_the_outermost_loop(batch_asm):
