@@ -1,4 +1,5 @@
-"""Builds libpvhip.so (the HIP backend) for gfx950 with hipcc, in-tree.
+"""Builds libpvhip.so (the HIP backend) for gfx950 with hipcc, and the run driver
+point_vortex_hip linked to it, in-tree.
 
 Usage: python -m pointvortex_amd.build [--jobs N]
 
@@ -45,6 +46,23 @@ def _compile(src):
     return obj, r.stderr
 
 
+DRIVER_SRC = PKG / "driver" / "point_vortex_hip.cpp"
+DRIVER = LIBDIR / "point_vortex_hip"
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
+
+
+def build_driver():
+    """The run driver (host C++ over the C-ABI), linked to libpvhip.so beside it."""
+    tmp = DRIVER.with_suffix(".tmp")
+    cmd = [CXX, "-std=c++20", "-O2", "-Wall", "-Wextra", f"-I{INCLUDE}", str(DRIVER_SRC),
+           f"-L{LIBDIR}", "-lpvhip", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build(jobs=None, verbose=False):
     BUILD.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
@@ -61,6 +79,7 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     installed = LIBDIR / "launch_table.txt"
     if TUNED.exists():
         shutil.copyfile(TUNED, installed)
