@@ -181,8 +181,8 @@ pv_status pv_evaluate_resident(pv_handle *h, int repeats);
 pv_status pv_synchronize(pv_handle *h);
 
 /* Launch shapes of the pair kernels (no reference counterpart; the CUDA backend used one
- * fixed shape, src/backend_cuda.cu:402-416).  A shape is targets per thread (1, 2 or 4; 0 =
- * automatic), source splits (0 = automatic), the LDS source tile (128 or 256; 0 = the
+ * fixed shape, src/backend_cuda.cu:402-416).  A shape is targets per thread (1, 2 or 4; 8 on the
+ * symmetric path of the infinite plane, 4 elsewhere; 0 = automatic), source splits (0 = automatic), the LDS source tile (128 or 256; 0 = the
  * precision's default) and whether whole-population evaluations use the symmetric path (each
  * unordered pair computed once; 0 or 1; both precisions).  Precedence: pv_set_launch /
  * pv_set_launch_shape (symmetric -1 = not forced), then the first matching rule of the handle's

@@ -2,7 +2,7 @@
 
 A shape is (targets per thread, source splits, LDS source tile, symmetric); its blocks per CU
 follow from the split count.  The symmetric path computes each unordered pair once over pairs
-of tiles; its candidates vary targets per thread and the tile.  For every geometry, precision and N-bucket the tuner uploads a seeded
+of tiles; its candidates vary targets per thread (up to 8 on the infinite plane) and the tile.  For every geometry, precision and N-bucket the tuner uploads a seeded
 population at the bucket's upper N, evaluates it once with the built-in shape (the
 reference result), then for every distinct candidate shape:
 
@@ -40,6 +40,7 @@ from .backend import HipKernel, SimParams, VortexSystem
 
 BUCKETS = (256, 1024, 4096, 16384, 65536, 262144)
 TPT = (1, 2, 4)
+SYM_TPT = (1, 2, 4, 8)     # 8 where the symmetric kernel has it (infinite plane), else 4
 TILES = (128, 256)
 SPLITS = (0, 1, 2, 4, 8, 16, 32, 64)
 GEOMETRIES = ("infinite", "periodic", "disk")
@@ -116,7 +117,7 @@ def tune_one(geometry, precision, n, log=_log):
     rejected = 0
     candidates = [(tpt, tile, splits, 0) for tpt, tile, splits in
                   itertools.product(TPT, TILES, SPLITS)]
-    candidates += [(tpt, tile, 0, 1) for tpt, tile in itertools.product(TPT, TILES)]
+    candidates += [(tpt, tile, 0, 1) for tpt, tile in itertools.product(SYM_TPT, TILES)]
     for tpt, tile, splits, sym in candidates:
         k.set_launch(tpt, splits, tile, sym)
         shape = k.resolved_launch(n, n)
@@ -137,7 +138,7 @@ def tune_one(geometry, precision, n, log=_log):
     k.close()
     fastest = min(timed.values())
     near = [s for s, ms in timed.items() if ms <= fastest * (1.0 + TIE)]
-    # fewest partial sums: a split shape writes splits x 2N, a symmetric one (N/(256 tpt)) x 2N
+    # fewest partial sums: a split shape writes splits x 2N, a symmetric one (N/tile) x 2N
     best_shape = min(near, key=lambda s: (partial_slots(s, n), timed[s]))
     best_ms = timed[best_shape]
     res = {"geometry": geometry, "precision": precision, "n": n,
@@ -175,7 +176,12 @@ def _confirm(k, timed, n):
 
 def partial_slots(shape, n):
     tpt, splits, _, sym = shape
-    return -(-n // (256 * tpt)) if sym else (splits if splits > 1 else 0)
+    return -(-n // sym_tile(tpt)) if sym else (splits if splits > 1 else 0)
+
+
+def sym_tile(tpt):
+    """Vortices per tile of the symmetric kernel: 256 threads x tpt, 128 threads at 8."""
+    return (128 if tpt >= 8 else 256) * tpt
 
 
 def tune(geometries=GEOMETRIES, precisions=PRECISIONS, buckets=BUCKETS, log=_log):

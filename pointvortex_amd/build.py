@@ -1,5 +1,5 @@
-"""Builds libpvhip.so (the HIP backend) for gfx950 with hipcc, and the run driver
-point_vortex_hip linked to it, in-tree.
+"""Builds libpvhip.so (the HIP backend) for gfx950 with hipcc, the run driver
+point_vortex_hip linked to it, and the VALU issue-cost probe pv_issue_probe, in-tree.
 
 Usage: python -m pointvortex_amd.build [--jobs N]
 
@@ -63,6 +63,21 @@ def build_driver():
     return DRIVER
 
 
+PROBE_SRC = CSRC / "pv_issue_probe.hip"
+PROBE = LIBDIR / "pv_issue_probe"
+
+
+def build_probe():
+    """The VALU issue-cost probe (a stand-alone HIP program, no library)."""
+    tmp = PROBE.with_suffix(".tmp")
+    cmd = [HIPCC, "-x", "hip", *COMMON, str(PROBE_SRC), "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"probe build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, PROBE)
+    return PROBE
+
+
 def build(jobs=None, verbose=False):
     BUILD.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
@@ -80,6 +95,7 @@ def build(jobs=None, verbose=False):
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_driver()
+    build_probe()
     installed = LIBDIR / "launch_table.txt"
     if TUNED.exists():
         shutil.copyfile(TUNED, installed)

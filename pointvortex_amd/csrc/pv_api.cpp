@@ -270,9 +270,15 @@ Shape resolve_shape(const pv_handle *h, size_t targets, size_t n) {
     s.tiles = sp.tiles;
     s.split_len = sp.split_len;
     s.nsplit = sp.nsplit;
-    const size_t tt = (size_t)pv::kBlock * s.tpt, nt = (n + tt - 1) / tt;
+    // The symmetric kernel also has 8 targets per thread on the infinite plane; elsewhere, and
+    // on the split path, a request for 8 runs at 4.
+    const int sym_tpt =
+        tpt >= 8 && pv::sym_tpt_supported(h->cfg.geometry, prec, 8) ? 8 : s.tpt;
+    const size_t tt = (size_t)pv::sym_tile(sym_tpt), nt = (n + tt - 1) / tt;
     s.sym = sym == 1 && pv::sym_supported(h->cfg.geometry, prec) && targets == n && n > 0 &&
             nt * 2 * n * sizeof(double) <= kSymPartialLimit;
+    if (s.sym)
+        s.tpt = sym_tpt;
     return s;
 }
 
@@ -332,8 +338,8 @@ void check_rules(const pv_launch_rule *rules, size_t count) {
             (r.precision != PV_FP64 && r.precision != PV_MIXED))
             raise(PV_ERR_INVALID_ARGUMENT, "launch rule names an unknown geometry or precision");
         if (r.targets_per_thread != 0 && r.targets_per_thread != 1 && r.targets_per_thread != 2 &&
-            r.targets_per_thread != 4)
-            raise(PV_ERR_INVALID_ARGUMENT, "targets per thread must be 0, 1, 2 or 4");
+            r.targets_per_thread != 4 && r.targets_per_thread != 8)
+            raise(PV_ERR_INVALID_ARGUMENT, "targets per thread must be 0, 1, 2, 4 or 8");
         if (r.source_splits < 0)
             raise(PV_ERR_INVALID_ARGUMENT, "source splits must be non-negative");
         if (r.tile != 0 && r.tile != 128 && r.tile != 256)
@@ -386,7 +392,7 @@ void record_events(pv_handle *h, hipEvent_t &ev0, hipEvent_t &ev1) {
 // the stage epilogue.  Both launches are bracketed by the profiling events.
 void evaluate_symmetric(pv_handle *h, const double *x, const double *y, const double *g,
                         size_t n, const Shape &sh, const pv::StageOp &op) {
-    const size_t tt = (size_t)pv::kBlock * sh.tpt, nt = (n + tt - 1) / tt;
+    const size_t tt = (size_t)pv::sym_tile(sh.tpt), nt = (n + tt - 1) / tt;
     ensure_partials(h, nt * 2 * n);
     pv::SymLaunch a{};
     a.x = x, a.y = y, a.g = g;
@@ -679,8 +685,8 @@ pv_status pv_set_launch(pv_handle *h, int targets_per_thread, int source_splits)
     return guard([&] {
         checked(h);
         if (targets_per_thread != 0 && targets_per_thread != 1 && targets_per_thread != 2 &&
-            targets_per_thread != 4)
-            raise(PV_ERR_INVALID_ARGUMENT, "targets per thread must be 0, 1, 2 or 4");
+            targets_per_thread != 4 && targets_per_thread != 8)
+            raise(PV_ERR_INVALID_ARGUMENT, "targets per thread must be 0, 1, 2, 4 or 8");
         if (source_splits < 0)
             raise(PV_ERR_INVALID_ARGUMENT, "source splits must be non-negative");
         h->tpt_override = targets_per_thread;

@@ -10,6 +10,8 @@
 //             them in split order, so every target's sum has a fixed order (deterministic).
 // The stage arithmetic of RK4 / DOPRI5 is fused into the epilogue of the evaluation that
 // produces the stage (StageOp), so an RK4 step is four launches and a DOPRI5 attempt seven.
+#include <type_traits>
+
 #include "pv_epilogue.hpp"
 #include "pv_sym.hpp"
 
@@ -199,6 +201,25 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
 
 // ------------------------------------------------------------------ symmetric evaluation
 
+// The infinite-plane policies, whose pair is short enough that the ring's data movement is a
+// sizeable share of a step: their symmetric kernels also have 8 targets per thread.  The FP64
+// one's ring reads its sources from a doubled window (win), and its diagonal blocks share one
+// reciprocal over two sources x TPT targets, as the split kernel does.  (In the mixed kernel
+// the window costs more than it saves: the compiler then loads each source in the step that
+// uses it instead of one step ahead.)
+template <class P>
+constexpr bool kInfPlane = std::is_same_v<P, InfF64> || std::is_same_v<P, InfMixed>;
+template <class P> constexpr bool kRingWindow = std::is_same_v<P, InfF64>;
+
+// Slot of source k of a staged chunk: with the window, each group of 64 sources is staged twice
+// in a row (group k / 64 at 128 (k / 64)), so that the ring's index needs no wrap.
+template <class P> __device__ __forceinline__ int win(int k) {
+    if constexpr (kRingWindow<P>)
+        return k + (k & ~63);
+    else
+        return k;
+}
+
 // Block -> tile pair (I <= J) in row-major triangular order: b = J (J + 1) / 2 + I.
 __device__ __forceinline__ void tile_pair(int b, int &I, int &J) {
     int j = (int)((__builtin_sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
@@ -210,46 +231,71 @@ __device__ __forceinline__ void tile_pair(int b, int &I, int &J) {
     I = b - j * (j + 1) / 2;
 }
 
+// Symmetric blocks of the infinite-plane policies: the nt (nt - 1) / 2 off-diagonal pairs (I < J)
+// first, row-major, then the nt diagonal tiles, which take less time there and so fill the end
+// of the launch.  The other policies' diagonal blocks take as long as the others (or longer):
+// they keep the row-major order over I <= J, which spreads them through the launch.
+template <class P> __device__ __forceinline__ void sym_tile_pair(int b, int nt, int &I, int &J) {
+    if constexpr (!kInfPlane<P>) {
+        tile_pair(b, I, J);
+        return;
+    }
+    const int off = (int)((long long)nt * (nt - 1) / 2);
+    if (b >= off) {
+        I = J = b - off;
+    } else {
+        tile_pair(b, I, J);   // b = J (J + 1) / 2 + I with I <= J, so (I, J + 1) is b's pair
+        ++J;
+    }
+}
+
+// At 8 targets per thread the kernel is held to 168 VGPRs (3 waves per SIMD) rather than the
+// 180 it would take at 2.
 template <class P, int TPT, int TILE>
-__global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) {
+__global__ __launch_bounds__(sym_block(TPT))
+__attribute__((amdgpu_waves_per_eu(TPT >= 8 ? 3 : 1))) void sym_kernel(SymLaunch a, GeoParams gp) {
     if (a.st->fail_epoch < a.epoch)
         return;
     using Acc = typename P::Acc;
-    __shared__ typename P::Src sh[TILE];
-    __shared__ double red[kBlock / 64][TILE][2];
+    __shared__ typename P::Src sh[(kRingWindow<P> ? 2 : 1) * TILE];
+    constexpr int B = sym_block(TPT);
+    __shared__ double red[B / 64][TILE][2];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int TT = kBlock * TPT;
+    constexpr int TT = B * TPT;
     int I, J;
-    tile_pair(blockIdx.x, I, J);
+    sym_tile_pair<P>(blockIdx.x, a.nt, I, J);
     const int t0 = I * TT;
     const int s0 = J * TT, s1 = min(a.n, s0 + TT);
     const size_t n = (size_t)a.n;
 
     double xi[TPT], yi[TPT];
     typename P::SymScalar gi[TPT];
-    int ti[TPT];
+    // Target t of the lane (-1 for padding), recomputed where needed rather than held.
+    const auto ti = [&](int t) {
+        const int i = t0 + t * B + tid;
+        return i < a.n ? i : -1;
+    };
     int near = 0, sym_near = 0;
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
-        const int i = t0 + t * kBlock + tid;
+        const int i = t0 + t * B + tid;
         const bool valid = i < a.n;
         const int ii = valid ? i : t0;
         xi[t] = a.x[ii];
         yi[t] = a.y[ii];
         gi[t] = valid ? P::sym_g(gp, a.g[ii]) : 0;       // a padding target adds no reaction
-        ti[t] = valid ? i : -1;
         if constexpr (P::kNearTiles)
             near |= valid && P::is_near(gp, xi[t], yi[t]);
         if constexpr (P::kSymNear)
             sym_near |= valid && P::is_sym_near(gp, xi[t], yi[t]);
     }
     if constexpr (P::kNearTiles) {
-        for (int k = tid; k < s1 - s0; k += kBlock)
+        for (int k = tid; k < s1 - s0; k += B)
             near |= P::is_near(gp, a.x[s0 + k], a.y[s0 + k]);
         near = __syncthreads_or(near);
     }
     if constexpr (P::kSymNear) {
-        for (int k = tid; k < s1 - s0; k += kBlock)
+        for (int k = tid; k < s1 - s0; k += B)
             sym_near |= P::is_sym_near(gp, a.x[s0 + k], a.y[s0 + k]);
         sym_near = __syncthreads_or(sym_near);
     }
@@ -268,20 +314,55 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
         for (int base = s0; base < s1; base += TILE) {
             const int cnt = min(TILE, s1 - base);
             __syncthreads();
-            for (int k = tid; k < cnt; k += kBlock)
+            for (int k = tid; k < cnt; k += B)
                 sh[k] = P::src(gp, a.x[base + k], a.y[base + k], a.g[base + k]);
             __syncthreads();
-            for (int k = 0; k < cnt; ++k) {
+            int k0 = 0;
+            if constexpr (kRingWindow<P>) {
+                // Q sources x TPT targets share one rcp64 (Q = 2, or 1 at 8 targets per thread).
+                // The self pair has zero difference and so no term; its denominator (and a
+                // padding lane's) is set to 1 so that the product stays usable.
+                constexpr int Q = TPT >= 8 ? 1 : 2, K = Q * TPT;
+                for (; k0 + Q <= cnt; k0 += Q) {
+                    typename P::Src sq[Q];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+                        sq[q] = sh[k0 + q];
+                    typename P::Tmp m[Q][TPT];
+                    double d[K], inv[K];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t) {
+                            P::prep(gp, tg[t], sq[q], m[q][t], &d[q * TPT + t]);
+                            const int it = ti(t);
+                            if (it < 0 || it == base + k0 + q)
+                                d[q * TPT + t] = 1.0;
+                        }
+                    const bool ok = TreeInverse<K>::run(d, inv);
+                    if (tree_fallback(ok)) {
+#pragma unroll
+                        for (int e = 0; e < K; ++e)
+                            inv[e] = rcp64(d[e]);
+                    }
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+#pragma unroll
+                        for (int t = 0; t < TPT; ++t)
+                            P::apply(sq[q], m[q][t], &inv[q * TPT + t], acc[t]);
+                }
+            }
+            for (int k = k0; k < cnt; ++k) {
                 const typename P::Src s_a = sh[k];
 #pragma unroll
                 for (int t = 0; t < TPT; ++t) {
                     if constexpr (P::kNearTiles) {
                         if (near)
-                            P::pair_ref_checked(gp, tg[t], s_a, acc[t], ti[t] == base + k);
+                            P::pair_ref_checked(gp, tg[t], s_a, acc[t], ti(t) == base + k);
                         else
-                            P::pair_checked(gp, tg[t], s_a, acc[t], ti[t] == base + k);
+                            P::pair_checked(gp, tg[t], s_a, acc[t], ti(t) == base + k);
                     } else {
-                        P::pair_checked(gp, tg[t], s_a, acc[t], ti[t] == base + k);
+                        P::pair_checked(gp, tg[t], s_a, acc[t], ti(t) == base + k);
                     }
                 }
             }
@@ -301,9 +382,14 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
             const int cnt = min(TILE, s1 - base);
             const int span = (cnt + 63) & ~63;          // <= TILE: TILE is a multiple of 64
             __syncthreads();
-            for (int k = tid; k < span; k += kBlock)
-                sh[k] = k < cnt ? ring_src<P>(gp, a.x[base + k], a.y[base + k], a.g[base + k])
-                                : ring_src<P>(gp, a.x[base], a.y[base], 0.0);
+            for (int k = tid; k < span; k += B) {
+                const typename P::Src v =
+                    k < cnt ? ring_src<P>(gp, a.x[base + k], a.y[base + k], a.g[base + k])
+                            : ring_src<P>(gp, a.x[base], a.y[base], 0.0);
+                sh[win<P>(k)] = v;
+                if constexpr (kRingWindow<P>)
+                    sh[win<P>(k) + 64] = v;
+            }
             __syncthreads();
             bool done = false;
             if constexpr (P::kNearTiles) {
@@ -314,10 +400,10 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
 #pragma unroll
                     for (int t = 0; t < TPT; ++t) {
                         tr[t] = P::tgt(gp, xi[t], yi[t]);
-                        si[t] = P::src(gp, xi[t], yi[t], ti[t] >= 0 ? a.g[ti[t]] : 0.0);
+                        si[t] = P::src(gp, xi[t], yi[t], ti(t) >= 0 ? a.g[ti(t)] : 0.0);
                     }
                     for (int k = 0; k < cnt; ++k) {
-                        const typename P::Src s_a = sh[k];
+                        const typename P::Src s_a = sh[win<P>(k)];
                         const typename P::Tgt tj = P::tgt(gp, s_a.x, s_a.y);
                         Acc r;
                         P::zero(r);
@@ -337,8 +423,9 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
             }
             // Ring: lane l starts on source g0 + l with that source's reaction in r; after each
             // step r moves one lane (wave_ror:1) and the lane reads the source whose reaction it
-            // now holds.  64 steps bring every reaction back to its first lane with the sum
-            // over the wave's targets.  Staged pads (strength 0) fill the last group of 64;
+            // now holds, (l + dir k) mod 64 after k steps (in the window: entry l + k for dir 1,
+            // l + 64 - k for dir 63).  64 steps bring every reaction back to its first lane with
+            // the sum over the wave's targets.  Staged pads (strength 0) fill the last group of 64;
             // their reactions are dropped.  `direct` is the near-centre form of kSymNear
             // policies (block-uniform, so each form is its own loop).
             const auto ring = [&](auto direct) {
@@ -347,13 +434,19 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
                     typename P::SymR r;
                     r.u = 0;
                     r.v = 0;
-                    typename P::Src next = sh[g0 + lane];
-#pragma unroll P::kRingUnroll
+                    const typename P::Src *w = &sh[win<P>(g0) + lane + (dir == 1 ? 0 : 64)];
+                    const int step = dir == 1 ? 1 : -1;
+                    typename P::Src next = sh[win<P>(g0) + lane];
+                    constexpr int kUnroll = TPT >= 8 ? 1 : P::kRingUnroll;
+#pragma unroll kUnroll
                     for (int s = 0; s < 64; ++s) {
                         // The next step's source is read one step ahead (at s = 63 this reads
                         // the lane's first source again, unused).
                         const typename P::Src s_a = next;
-                        next = sh[g0 + ((lane + dir * (s + 1)) & 63)];
+                        if constexpr (kRingWindow<P>)
+                            next = w[step * (s + 1)];
+                        else
+                            next = sh[g0 + ((lane + dir * (s + 1)) & 63)];
                         sym_step<P, TPT, kDirect>(gp, tg, gi, s_a, acc, r);
                         r.u = wave_ror1(r.u);
                         r.v = wave_ror1(r.v);
@@ -376,11 +469,11 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
                     P::fold(acc[t]);
             }
             __syncthreads();
-            // The reaction on this chunk's sources: the four waves in order, into slot I.
-            for (int k = tid; k < cnt; k += kBlock) {
+            // The reaction on this chunk's sources: the block's waves in order, into slot I.
+            for (int k = tid; k < cnt; k += B) {
                 double ru = red[0][k][0], rv = red[0][k][1];
 #pragma unroll
-                for (int w = 1; w < kBlock / 64; ++w) {
+                for (int w = 1; w < B / 64; ++w) {
                     ru += red[w][k][0];
                     rv += red[w][k][1];
                 }
@@ -392,11 +485,11 @@ __global__ __launch_bounds__(kBlock) void sym_kernel(SymLaunch a, GeoParams gp) 
     // The target side into slot J (slot I on the diagonal, where J == I).
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
-        if (ti[t] >= 0) {
+        if (ti(t) >= 0) {
             double u, v;
             P::result(acc[t], u, v);
-            a.part[(size_t)(2 * J) * n + ti[t]] = u;
-            a.part[(size_t)(2 * J + 1) * n + ti[t]] = v;
+            a.part[(size_t)(2 * J) * n + ti(t)] = u;
+            a.part[(size_t)(2 * J + 1) * n + ti(t)] = v;
         }
     }
 }
@@ -797,15 +890,27 @@ bool sym_supported(int geometry, int precision) {
     return geometry >= 0 && geometry <= 2 && (precision == 0 || precision == 1);
 }
 
+bool sym_tpt_supported(int geometry, int precision, int tpt) {
+    if (tpt == 1 || tpt == 2 || tpt == 4)
+        return sym_supported(geometry, precision);
+    return tpt == 8 && geometry == 0 && (precision == 0 || precision == 1);
+}
+
 template <class P, int TILE>
 static void launch_sym_tpt(int tpt, const SymLaunch &a, const GeoParams &gp, hipStream_t s) {
     const dim3 grid((unsigned)((size_t)a.nt * (a.nt + 1) / 2));
+    if constexpr (kInfPlane<P>) {
+        if (tpt >= 8) {
+            hipLaunchKernelGGL((sym_kernel<P, 8, TILE>), grid, dim3(sym_block(8)), 0, s, a, gp);
+            return;
+        }
+    }
     if (tpt >= 4)
-        hipLaunchKernelGGL((sym_kernel<P, 4, TILE>), grid, dim3(kBlock), 0, s, a, gp);
+        hipLaunchKernelGGL((sym_kernel<P, 4, TILE>), grid, dim3(sym_block(4)), 0, s, a, gp);
     else if (tpt == 2)
-        hipLaunchKernelGGL((sym_kernel<P, 2, TILE>), grid, dim3(kBlock), 0, s, a, gp);
+        hipLaunchKernelGGL((sym_kernel<P, 2, TILE>), grid, dim3(sym_block(2)), 0, s, a, gp);
     else
-        hipLaunchKernelGGL((sym_kernel<P, 1, TILE>), grid, dim3(kBlock), 0, s, a, gp);
+        hipLaunchKernelGGL((sym_kernel<P, 1, TILE>), grid, dim3(sym_block(1)), 0, s, a, gp);
 }
 
 template <class P>

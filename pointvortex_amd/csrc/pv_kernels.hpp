@@ -55,7 +55,7 @@ struct PairLaunch {
 };
 
 // Symmetric evaluation of a whole population (every geometry, FP64 and mixed): one block per
-// unordered pair of tiles (I <= J) of kBlock*TPT vortices.  Off-diagonal blocks compute each
+// unordered pair of tiles (I <= J) of sym_tile(TPT) vortices.  Off-diagonal blocks compute each
 // pair once and write the target side (into tile I's vortices, slot J) and the reaction (into
 // tile J's vortices, slot I); diagonal blocks compute the ordered pairs inside their tile (slot
 // I).  Every (slot, vortex) entry of `part` is written exactly once; the reduce kernel then sums
@@ -63,7 +63,7 @@ struct PairLaunch {
 struct SymLaunch {
     const double *x, *y, *g;
     int n;
-    int nt;                    // tiles of kBlock*TPT vortices
+    int nt;                    // tiles of sym_tile(TPT) vortices
     double *part;              // [nt][2][n]
     DevStatus *st;
     unsigned epoch;
@@ -71,6 +71,12 @@ struct SymLaunch {
 };
 
 constexpr int kBlock = 256;
+
+// Threads of a symmetric block: kBlock, except 128 at 8 targets per thread, so that a tile
+// stays at 1024 vortices and N = 65536 still gives 2080 blocks.  8 targets per thread serve the
+// infinite-plane policies only (sym_tpt_supported).
+constexpr int sym_block(int tpt) { return tpt >= 8 ? 128 : kBlock; }
+constexpr int sym_tile(int tpt) { return sym_block(tpt) * tpt; }
 constexpr int kTileF64 = 256;
 constexpr int kTileMixed = 128;
 
@@ -95,7 +101,7 @@ struct RkPersistLaunch {
     StageOp ops[8];
 };
 
-// Launchers (pv_kernels.hip).  `tpt` is 1, 2 or 4.
+// Launchers (pv_kernels.hip).  `tpt` is 1, 2 or 4 (launch_sym: also 8 where sym_tpt_supported).
 // tile: sources staged in LDS per step, 128 or 256 (pair_tile() gives each precision's default).
 void launch_pair(int geometry, int precision, int tpt, int tile, const PairLaunch &a,
                  const GeoParams &gp, hipStream_t s);
@@ -103,6 +109,7 @@ int pair_tile(int precision);
 // Symmetric path (every geometry and precision): two launches, the pairs then the
 // reduce + epilogue.
 bool sym_supported(int geometry, int precision);
+bool sym_tpt_supported(int geometry, int precision, int tpt);
 void launch_sym(int geometry, int precision, int tpt, int tile, const SymLaunch &a,
                 const GeoParams &gp, hipStream_t s);
 void launch_sym_reduce(const SymLaunch &a, const GeoParams &gp, hipStream_t s);

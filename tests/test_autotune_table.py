@@ -41,4 +41,4 @@ def test_malformed_tables_are_refused(tmp_path, text):
 def test_committed_table_parses():
     if autotune.TABLE.exists():
         rules = autotune.read_table(autotune.TABLE)
-        assert all(r[3] in (1, 2, 4) and r[5] in (128, 256) and r[6] in (0, 1) for r in rules)
+        assert all(r[3] in (1, 2, 4, 8) and r[5] in (128, 256) and r[6] in (0, 1) for r in rules)

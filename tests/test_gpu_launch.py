@@ -140,6 +140,45 @@ def test_symmetric_mixed_keeps_parity(oracle, geometry, n, tpt, tile):
     assert err <= max(3.0 * err_split, 1e-7), (err, err_split)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n", [2, 700, 1500, 5000])
+@pytest.mark.parametrize("tile", [128, 256])
+def test_symmetric_eight_targets_per_thread(oracle, precision, n, tile):
+    """8 targets per thread on the infinite plane: 128-thread blocks over 1024-vortex tiles, the
+    product tree over 8 denominators.  FP64 within the rounding gate of the oracle; mixed within
+    1e-6 and no less accurate than the 4-target symmetric kernel's sums."""
+    pop, k = setup("infinite", precision, n, seed=21 + n)
+    s = VortexSystem(pop.x, pop.y, pop.circulation)
+    k.set_launch(8, 0, tile, 1)
+    assert k.resolved_launch(pop.n, pop.n) == (8, 0, tile, 1)
+    vel = k.evaluate(s)
+    ur, vr = oracle.evaluate(spec("infinite"), pop.x, pop.y, pop.circulation)
+    err = normwise(vel.x, vel.y, ur, vr)
+    if precision == "fp64":
+        assert err <= fp64_gate("infinite", pop.x, pop.y, pop.circulation, ur, vr)
+    else:
+        k.set_launch(4, 0, tile, 1)
+        four = k.evaluate(s)
+        assert err <= 1e-6
+        assert err <= max(3.0 * normwise(four.x, four.y, ur, vr), 1e-7)
+
+
+@pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
+def test_eight_targets_per_thread_elsewhere_run_at_four(geometry):
+    """Only the infinite plane's symmetric kernel has 8 targets per thread: a request for 8 on
+    the split path, or on the other geometries, resolves to (and runs at) 4."""
+    pop, k = setup(geometry, "fp64", 700, seed=3)
+    k.set_launch(8, 3, 128, 0)
+    assert k.resolved_launch(pop.n, pop.n) == (4, 3, 128, 0)
+    if geometry != "infinite":
+        k.set_launch(8, 0, 128, 1)
+        assert k.resolved_launch(pop.n, pop.n) == (4, 0, 128, 1)
+        vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+        k.set_launch(4, 0, 128, 1)
+        four = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+        assert np.array_equal(vel.x, four.x) and np.array_equal(vel.y, four.y)
+
+
 @pytest.mark.parametrize("pos", [(2e-160, 1e-161), (-3e-31, 2e-31), (6e-13, -4e-13),
                                  (8e-13, 5e-13), (3e-7, 1e-7)])
 def test_symmetric_mixed_near_centre_disk_source(oracle, pos):
