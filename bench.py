@@ -198,8 +198,9 @@ def ensemble_bench(pv, steps, count=4096):
         out[prec] = {"wall_s": wall, "solves_per_s": solved / wall, "pairs_per_s": pairs / wall,
                      "completed_instances": solved, "failed_instances": count - solved,
                      "failures": reasons,
-                     "decomposition": ("symmetric: pairs of 128-vortex tiles, each unordered "
-                                       "pair once" if e.symmetric else
+                     "decomposition": ("symmetric: pairs of tiles (64, 128 or 256 vortices by "
+                                       "the ensemble launch rules), each unordered pair once"
+                                       if e.symmetric else
                                        "ordered: 64-target items against all sources")}
         log(f"[bench] ensemble {prec}: {wall:.2f} s, {solved / wall:.1f} solves/s, "
             f"{pairs / wall:.3e} pairs/s, failed {count - solved}: {reasons}")

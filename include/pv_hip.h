@@ -301,19 +301,42 @@ pv_status pv_batch_set_byte_limit(pv_batch *b, size_t bytes);
 pv_status pv_batch_device_bytes(pv_batch *b, size_t *bytes);
 
 /* How an ensemble evaluates its pairs (no reference counterpart; the reference runs one
- * system per process).  mode 1: symmetric -- work items are pairs of 128-vortex tiles of one
- * instance, each unordered pair computed once, with per-instance partial sums (16 B per
- * vortex per 128 vortices of its instance, on top of the 208 B per vortex) reduced in a fixed
- * order; mode 0: ordered -- 64-target items against all of the instance's sources; mode -1
- * (default): symmetric for fixed-step ensembles (pv_batch_load), ordered for admitted runs,
- * whose admission budget counts 208 B per vortex.  Both give every instance a result that
- * does not depend on the rest of the ensemble.  pv_batch_symmetric reports the mode the
- * current work items use. */
+ * system per process).  mode 1: symmetric -- work items are pairs of tiles of one instance
+ * (64, 128 or 256 vortices, per pv_batch_set_launch_table), each unordered pair computed once,
+ * with per-instance partial sums (16 B per vortex per tile of its instance, on top of the
+ * 208 B per vortex) reduced in a fixed order; mode 0: ordered -- 64-target items against all
+ * of the instance's sources; mode -1 (default): symmetric for fixed-step ensembles
+ * (pv_batch_load), ordered for admitted runs, whose admission budget counts 208 B per vortex.
+ * Both give every instance a result that does not depend on the rest of the ensemble.
+ * pv_batch_symmetric reports the mode the current work items use. */
 pv_status pv_batch_set_symmetric(pv_batch *b, int mode);
 int pv_batch_symmetric(const pv_batch *b);
-/* Partial-sum bytes the symmetric decomposition allocates for one instance of n vortices,
- * on top of n x pv_batch_bytes_per_vortex(). */
-size_t pv_batch_symmetric_bytes(size_t n);
+
+/* Launch shapes of the symmetric decomposition (no reference counterpart): per (geometry,
+ * precision, N-bucket), the targets each lane holds (1, 2 or 4: tiles of 64, 128 or 256
+ * vortices; 0 = 2) and the most work items a wave takes per counter increment (1..64; 0 = 4).
+ * The first rule covering an instance (n <= n_max) decides; instances no rule covers take the
+ * defaults.  The items of each (tile, geometry) group run in a launch of their own.  A new
+ * ensemble reads its rules from $PV_HIP_ENSEMBLE_TABLE, else from ensemble_table.txt beside
+ * libpvhip.so when present (lines "geometry precision n_max targets_per_lane chunk_items", as
+ * written by pointvortex_amd/autotune.py --ensemble).  Results depend on an instance's own
+ * rule only: a run gives the same bits alone and in a mix.  pv_batch_resolve_launch reports
+ * the shape an instance would take; pv_batch_instance_bytes the device bytes one admitted run
+ * adds beyond n x pv_batch_bytes_per_vortex() (work items, and with sym != 0 the symmetric
+ * items and partial sums at its tile) -- what admission by byte limit counts. */
+typedef struct pv_batch_rule {
+    int geometry;            /* pv_geometry */
+    int precision;           /* pv_precision */
+    size_t n_max;            /* applies to instances of at most n_max vortices */
+    int targets_per_lane;
+    int chunk_items;
+} pv_batch_rule;
+
+pv_status pv_batch_set_launch_table(pv_batch *b, const pv_batch_rule *rules, size_t count);
+pv_status pv_batch_resolve_launch(pv_batch *b, int geometry, int precision, size_t n,
+                                  int *targets_per_lane, int *chunk_items);
+pv_status pv_batch_instance_bytes(pv_batch *b, int geometry, int precision, size_t n, int sym,
+                                  size_t *bytes);
 
 /* Fixed-step ensembles: replace the contents with `ninst` instances laid out one after the
  * other in x, y, gamma (counts[k] vortices each), advance them with RK4 (optionally as

@@ -101,6 +101,16 @@ class PvLaunchRule(ctypes.Structure):
     ]
 
 
+class PvBatchRule(ctypes.Structure):
+    _fields_ = [
+        ("geometry", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("n_max", ctypes.c_size_t),
+        ("targets_per_lane", ctypes.c_int),
+        ("chunk_items", ctypes.c_int),
+    ]
+
+
 class PvSlotState(ctypes.Structure):
     _fields_ = [
         ("slot", ctypes.c_int),
@@ -176,7 +186,13 @@ _SIGNATURES = {
     "pv_dipole_random_state": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t,
                                               ctypes.POINTER(ctypes.c_size_t)]),
     "pv_batch_symmetric": (ctypes.c_int, [_vp]),
-    "pv_batch_symmetric_bytes": (ctypes.c_size_t, [ctypes.c_size_t]),
+    "pv_batch_set_launch_table": (ctypes.c_int, [_vp, ctypes.POINTER(PvBatchRule),
+                                                 ctypes.c_size_t]),
+    "pv_batch_resolve_launch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                               ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_int)]),
+    "pv_batch_instance_bytes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                               ctypes.c_int, _szp]),
     "pv_batch_bytes_per_vortex": (ctypes.c_size_t, []),
     "pv_batch_load": (ctypes.c_int, [_vp, ctypes.POINTER(PvInstanceConfig), _szp,
                                      ctypes.c_size_t, _dp, _dp, _dp]),

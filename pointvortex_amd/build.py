@@ -22,6 +22,7 @@ LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libpvhip.so"
 # Tuned launch shapes (pointvortex_amd.autotune); the library reads launch_table.txt beside it.
 TUNED = PKG / "tuning" / "launch_table.txt"
+TUNED_ENSEMBLE = PKG / "tuning" / "ensemble_table.txt"
 INCLUDE = ROOT / "include"
 
 ARCH = os.environ.get("PV_OFFLOAD_ARCH", "gfx950")
@@ -96,11 +97,12 @@ def build(jobs=None, verbose=False):
     os.replace(tmp, LIB)
     build_driver()
     build_probe()
-    installed = LIBDIR / "launch_table.txt"
-    if TUNED.exists():
-        shutil.copyfile(TUNED, installed)
-    elif installed.exists():
-        installed.unlink()
+    for tuned, name in ((TUNED, "launch_table.txt"), (TUNED_ENSEMBLE, "ensemble_table.txt")):
+        installed = LIBDIR / name
+        if tuned.exists():
+            shutil.copyfile(tuned, installed)
+        elif installed.exists():
+            installed.unlink()
     return LIB
 
 

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(BatchLaunch a) {
     for (;;) {
         int item = 0;
         if (lane == 0)
-            item = (int)atomicAdd(a.counter, 1u);
+            item = (int)atomicAdd(a.next, 1u);
         item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
         if (item >= a.nitems)
             break;
@@ -210,30 +210,95 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(BatchLaunch a) {
 
 // ------------------------------------------------------------------ symmetric mode
 
-// Per-wave LDS of the symmetric item: the kSymTile source records of tile J, and the reaction
-// sums of the near-centre disk path (written by every lane with the same value, read back one
-// source per lane, so no lane-0-only store sits inside the persistent loop).
-template <class P> struct SymSlice {
-    typename P::Src src[kSymTile];
-    double red[kSymTile][2];
+// Per-wave LDS of the symmetric item: the 64 TPT source records of tile J, and for the
+// near-tile policies the reaction sums of their near path (written by every lane with the same
+// value, read back one source per lane, so no lane-0-only store sits inside the persistent
+// loop).
+template <class P, int TPT, bool RED = P::kNearTiles> struct SymSlice {
+    typename P::Src src[64 * TPT];
+    double red[64 * TPT][2];
+};
+template <class P, int TPT> struct SymSlice<P, TPT, false> {
+    typename P::Src src[64 * TPT];
 };
 
-// LDS bytes of one wave's slice, the largest of a set's policies (0 for NoSet).
-template <class S> constexpr size_t set_slice_bytes() {
+// The policy of geometry GEO (0..2) in a set.
+template <class S, int GEO>
+using GeoPolicy = std::conditional_t<GEO == 0, typename S::I,
+                                     std::conditional_t<GEO == 1, typename S::P, typename S::D>>;
+
+// Tile widths of the per-instance kernel (TPT = 0 below), which takes each item at its
+// instance's width: 1 and 2 targets per lane for every policy, and 4 for the infinite plane,
+// whose 4-target code fits the kernel's registers.  The periodic and disk 4-target items run
+// in kernels of their own (one geometry, one width: TPT = 4).
+template <class P>
+constexpr bool kWide4 = std::is_same<P, InfF64>::value || std::is_same<P, InfMixed>::value;
+
+template <class P> constexpr size_t per_instance_slice_bytes() {
+    const size_t narrow = sizeof(SymSlice<P, 1>) > sizeof(SymSlice<P, 2>) ? sizeof(SymSlice<P, 1>)
+                                                                          : sizeof(SymSlice<P, 2>);
+    if constexpr (kWide4<P>)
+        return narrow > sizeof(SymSlice<P, 4>) ? narrow : sizeof(SymSlice<P, 4>);
+    else
+        return narrow;
+}
+
+// LDS bytes of one wave's slice: geometry GEO of a set at width TPT, or (TPT = 0, every
+// geometry) the largest the per-instance kernel needs; 0 for NoSet.
+template <class S, int TPT, int GEO> constexpr size_t set_slice_bytes() {
     if constexpr (std::is_same<S, NoSet>::value)
         return 0;
+    else if constexpr (TPT == 0)
+        return max3(per_instance_slice_bytes<typename S::I>(),
+                    per_instance_slice_bytes<typename S::P>(),
+                    per_instance_slice_bytes<typename S::D>());
     else
-        return max3(sizeof(SymSlice<typename S::I>), sizeof(SymSlice<typename S::P>),
-                    sizeof(SymSlice<typename S::D>));
+        return sizeof(SymSlice<GeoPolicy<S, GEO>, TPT>);
+}
+
+template <class P, int TPT>
+__device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const PairItem &it,
+                               SymSlice<P, TPT> *sl, int lane, int dir);
+
+// One item with the set's policy of geometry GEO at width TPT, or (TPT = 0, GEO =
+// kSymAllGeometries) with its instance's geometry and width.
+template <class S, int TPT, int GEO>
+__device__ __forceinline__ void sym_item_of(const BatchLaunch &a, const Instance *ins,
+                                            const PairItem &it, unsigned char *mine, int lane,
+                                            int dir) {
+    if constexpr (TPT == 0) {
+        const int geometry = ins->gp.geometry;
+        if (geometry == 0)
+            sym_item_of<S, -1, 0>(a, ins, it, mine, lane, dir);
+        else if (geometry == 1)
+            sym_item_of<S, -1, 1>(a, ins, it, mine, lane, dir);
+        else
+            sym_item_of<S, -1, 2>(a, ins, it, mine, lane, dir);
+    } else if constexpr (TPT == -1) {
+        // The instance's width; the host never gives this kernel a 4-target item of a policy
+        // without kWide4 (those run in their own launch), so such an item is skipped here.
+        using P = GeoPolicy<S, GEO>;
+        const int tpt = ins->tpt;
+        if (tpt == 1) {
+            batch_sym_item<P, 1>(a, ins, it, reinterpret_cast<SymSlice<P, 1> *>(mine), lane, dir);
+        } else if (tpt == 2) {
+            batch_sym_item<P, 2>(a, ins, it, reinterpret_cast<SymSlice<P, 2> *>(mine), lane, dir);
+        } else if constexpr (kWide4<P>) {
+            batch_sym_item<P, 4>(a, ins, it, reinterpret_cast<SymSlice<P, 4> *>(mine), lane, dir);
+        }
+    } else {
+        using P = GeoPolicy<S, GEO>;
+        batch_sym_item<P, TPT>(a, ins, it, reinterpret_cast<SymSlice<P, TPT> *>(mine), lane, dir);
+    }
 }
 
 // One pair of tiles (I, J) of one instance: partial slot J of tile I's targets, and for I < J
 // slot I of tile J's sources (the reaction).  I == J: ordered pairs inside the tile.
-template <class P>
+template <class P, int TPT>
 __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const PairItem &it,
-                               SymSlice<P> *sl, int lane, int dir) {
+                               SymSlice<P, TPT> *sl, int lane, int dir) {
     using Acc = typename P::Acc;
-    constexpr int TPT = kSymTpt, TT = kSymTile;
+    constexpr int TT = 64 * TPT;
     const GeoParams &gp = ins->gp;
     const int off = ins->offset, n = ins->n;
     const size_t nn = (size_t)n;
@@ -384,9 +449,13 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
     }
 }
 
-template <class F, class M, bool ADAPT>
+// The items of one launch group: TPT = 0 for the per-instance kernel (every geometry, each
+// item at its instance's width), or TPT = 4 with GEO its one geometry (the periodic or disk
+// 4-target items).  It dispatches on precision when the launch carries both sets.
+template <class F, class M, bool ADAPT, int TPT, int GEO>
 __global__ __launch_bounds__(kBatchBlock) void batch_sym_kernel(BatchLaunch a) {
-    constexpr size_t kSliceBytes = max3(set_slice_bytes<F>(), set_slice_bytes<M>(), 0);
+    constexpr size_t kSliceBytes =
+        max3(set_slice_bytes<F, TPT, GEO>(), set_slice_bytes<M, TPT, GEO>(), 0);
     __shared__ __align__(16) unsigned char lds[kBatchBlock / 64][kSliceBytes];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     unsigned char *mine = lds[wave];
@@ -397,7 +466,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_sym_kernel(BatchLaunch a) {
         // counter.
         int chunk = 0;
         if (lane == 0)
-            chunk = (int)atomicAdd(a.counter, 1u);
+            chunk = (int)atomicAdd(a.next, 1u);
         chunk = __builtin_amdgcn_readfirstlane(__shfl(chunk, 0, 64));
         if (chunk >= a.nchunks)
             break;
@@ -412,36 +481,15 @@ __global__ __launch_bounds__(kBatchBlock) void batch_sym_kernel(BatchLaunch a) {
                 continue;
             if (ADAPT && (sp.state != kRunning || (a.pass == kPassNeedK0 && sp.fsal)))
                 continue;
-            const int geometry = ins->gp.geometry;
             bool done = false;
             if constexpr (!std::is_same<M, NoSet>::value) {
                 if (ins->precision) {
-                    if (geometry == 0)
-                        batch_sym_item<typename M::I>(
-                            a, ins, it, reinterpret_cast<SymSlice<typename M::I> *>(mine), lane,
-                            dir);
-                    else if (geometry == 1)
-                        batch_sym_item<typename M::P>(
-                            a, ins, it, reinterpret_cast<SymSlice<typename M::P> *>(mine), lane,
-                            dir);
-                    else
-                        batch_sym_item<typename M::D>(
-                            a, ins, it, reinterpret_cast<SymSlice<typename M::D> *>(mine), lane,
-                            dir);
+                    sym_item_of<M, TPT, GEO>(a, ins, it, mine, lane, dir);
                     done = true;
                 }
             }
-            if (!done) {
-                if (geometry == 0)
-                    batch_sym_item<typename F::I>(
-                        a, ins, it, reinterpret_cast<SymSlice<typename F::I> *>(mine), lane, dir);
-                else if (geometry == 1)
-                    batch_sym_item<typename F::P>(
-                        a, ins, it, reinterpret_cast<SymSlice<typename F::P> *>(mine), lane, dir);
-                else
-                    batch_sym_item<typename F::D>(
-                        a, ins, it, reinterpret_cast<SymSlice<typename F::D> *>(mine), lane, dir);
-            }
+            if (!done)
+                sym_item_of<F, TPT, GEO>(a, ins, it, mine, lane, dir);
         }
     }
 }
@@ -498,7 +546,9 @@ __global__ __launch_bounds__(kBatchBlock) void batch_sym_reduce_kernel(BatchLaun
 
 __global__ void batch_reset_kernel(unsigned *counter) {
     if (threadIdx.x == 0)
-        *counter = 0u;
+        counter[0] = 0u;
+    else if (threadIdx.x <= kSymGroups)
+        counter[1 + threadIdx.x] = 0u;
 }
 
 // One fixed RK4 step finished (its last stage launch precedes this on the stream).
@@ -615,12 +665,44 @@ void launch_batch(int precisions, bool adaptive, const BatchLaunch &a, int block
     PV_LAUNCH_SETS(batch_kernel, precisions, adaptive, a, blocks, s);
 }
 
-void launch_batch_sym(int precisions, bool adaptive, const BatchLaunch &a, int blocks,
-                      hipStream_t s) {
-    PV_LAUNCH_SETS(batch_sym_kernel, precisions, adaptive, a, blocks, s);
+#undef PV_LAUNCH_SETS
+
+namespace {
+
+template <class F, class M, int TPT, int GEO>
+void launch_sym_group(bool adaptive, const BatchLaunch &a, int blocks, hipStream_t s) {
+    const dim3 grid(blocks), block(kBatchBlock);
+    if (adaptive)
+        hipLaunchKernelGGL((batch_sym_kernel<F, M, true, TPT, GEO>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((batch_sym_kernel<F, M, false, TPT, GEO>), grid, block, 0, s, a);
 }
 
-#undef PV_LAUNCH_SETS
+} // namespace
+
+namespace {
+
+template <class F, class M>
+void launch_sym_set(int group, bool adaptive, const BatchLaunch &a, int blocks, hipStream_t s) {
+    if (group == 1)
+        launch_sym_group<F, M, 4, 1>(adaptive, a, blocks, s);
+    else if (group == 2)
+        launch_sym_group<F, M, 4, 2>(adaptive, a, blocks, s);
+    else
+        launch_sym_group<F, M, 0, kSymAllGeometries>(adaptive, a, blocks, s);
+}
+
+} // namespace
+
+void launch_batch_sym(int precisions, bool adaptive, int group, const BatchLaunch &a, int blocks,
+                      hipStream_t s) {
+    if (precisions == 3)
+        launch_sym_set<F64Set, MixedSet>(group, adaptive, a, blocks, s);
+    else if (precisions == 2)
+        launch_sym_set<MixedSet, NoSet>(group, adaptive, a, blocks, s);
+    else
+        launch_sym_set<F64Set, NoSet>(group, adaptive, a, blocks, s);
+}
 
 void launch_batch_sym_reduce(bool adaptive, const BatchLaunch &a, hipStream_t s) {
     const dim3 grid((unsigned)((a.span + kBatchBlock - 1) / kBatchBlock)), block(kBatchBlock);

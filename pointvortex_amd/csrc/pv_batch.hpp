@@ -6,7 +6,8 @@
 namespace pv {
 
 // One instance of the ensemble: its vortices are [offset, offset + n) of the flat SoA arrays.
-// Symmetric mode: the instance's partial sums are nt tile rows x 2n doubles at part + poff.
+// Symmetric mode: the instance's tiles are 64 tpt vortices, and its partial sums are nt tile
+// rows x 2n doubles at part + poff.
 struct Instance {
     GeoParams gp;
     int offset;
@@ -14,6 +15,8 @@ struct Instance {
     int nt;
     int precision;   // the instance's pair arithmetic: 0 FP64, 1 mixed
     size_t poff;
+    int tpt;         // symmetric mode: targets per lane (1, 2 or 4)
+    int pad;
 };
 
 // One unit of work: 64 targets [tbegin, tend) (instance-local) of one instance against all
@@ -45,12 +48,17 @@ struct InstanceStep {
     const double *evaluated[2];       // {x, y} the first failing evaluation read
 };
 
-// Symmetric mode: one unit of work is a pair of kSymTile-vortex tiles (I <= J) of one
-// instance.  One wave computes each of its unordered pairs once and writes the reaction on
-// tile J into partial slot I and tile I's side into slot J; every (slot, vortex) is written
-// by exactly one item, and a reduction adds a vortex's slots in slot order.
-constexpr int kSymTpt = 2;
-constexpr int kSymTile = 64 * kSymTpt;
+// Symmetric mode: one unit of work is a pair of tiles (I <= J) of one instance, 64 tpt
+// vortices each, tpt its instance's targets per lane.  One wave computes each of its unordered
+// pairs once and writes the reaction on tile J into partial slot I and tile I's side into slot
+// J; every (slot, vortex) is written by exactly one item, and a reduction adds a vortex's slots
+// in slot order.  One launch (group 0) takes every item at its instance's width, except the
+// periodic and disk 4-target items, which run in launches of their own (groups 1 and 2: their
+// 4-target code needs more registers than the shared kernel should hold).
+constexpr int kSymTptDefault = 2;
+constexpr int kSymGroups = 3;
+constexpr int kSymAllGeometries = 3;
+inline int sym_group(int tpt, int geometry) { return tpt >= 4 && geometry != 0 ? geometry : 0; }
 // A wave takes one chunk of consecutive items per counter increment: the host cuts the
 // cost-sorted items into runs of about equal cost (the largest items alone), so the counter
 // sees far fewer atomics than items and the tail stays balanced.
@@ -77,8 +85,10 @@ struct BatchLaunch {
     const Instance *inst;
     const WorkItem *items;
     int nitems;
-    unsigned *counter;         // [0] next item, zeroed before every launch; [1] RK4 steps
-                               // completed since the ensemble was loaded
+    unsigned *counter;         // [0] next ordered item, [2 + g] next chunk of symmetric group g
+                               // (zeroed before every stage); [1] RK4 steps completed since
+                               // the ensemble was loaded
+    unsigned *next;            // this launch's item (ordered) or chunk (symmetric) counter
     const double *x, *y, *g;   // positions evaluated (flat) and circulations
     DevStatus *st;
     unsigned epoch;
@@ -97,10 +107,11 @@ struct BatchLaunch {
 // Launchers (pv_batch.hip).  `precisions` says which pair arithmetic the work items' instances
 // use: bit 0 FP64, bit 1 mixed (both: one launch dispatches on each instance's precision).
 void launch_batch(int precisions, bool adaptive, const BatchLaunch &a, int blocks, hipStream_t s);
+// Zeroes counter[0] and the kSymGroups chunk counters.
 void launch_batch_reset(unsigned *counter, hipStream_t s);
-// Symmetric mode: the pair kernel over PairItems, then the per-vortex reduction with the
-// stage epilogue.
-void launch_batch_sym(int precisions, bool adaptive, const BatchLaunch &a, int blocks,
+// Symmetric mode: the pair kernel over the PairItems of one launch group (sym_group), then
+// (once per stage) the per-vortex reduction with the stage epilogue.
+void launch_batch_sym(int precisions, bool adaptive, int group, const BatchLaunch &a, int blocks,
                       hipStream_t s);
 void launch_batch_sym_reduce(bool adaptive, const BatchLaunch &a, hipStream_t s);
 void launch_batch_step_done(unsigned *counter, hipStream_t s);

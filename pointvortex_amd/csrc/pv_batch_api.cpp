@@ -10,18 +10,101 @@
 
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
 using namespace pv::host;
 
 namespace {
+
+// Work counters: [0] next ordered item, [1] RK4 steps since the load, [2 + g] next chunk of
+// symmetric group g.
+constexpr size_t kCounterWords = 2 + pv::kSymGroups;
+
+// Ensemble launch table text: one rule per line, "geometry precision n_max targets_per_lane
+// chunk_items" (geometry infinite|periodic|disk, precision fp64|mixed); '#' starts a comment.
+// A malformed table is an error, not a silent fallback.
+std::vector<pv_batch_rule> read_batch_table(const std::string &path) {
+    std::ifstream in(path);
+    if (!in)
+        raise(PV_ERR_INVALID_ARGUMENT, "cannot read ensemble launch table " + path);
+    std::vector<pv_batch_rule> rules;
+    std::string line;
+    int lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos)
+            line.resize(hash);
+        std::istringstream ss(line);
+        std::string geo, prec, extra;
+        if (!(ss >> geo))
+            continue;
+        pv_batch_rule r{};
+        unsigned long long n_max = 0;
+        const std::string where =
+            "ensemble launch table " + path + ": line " + std::to_string(lineno);
+        if (!(ss >> prec >> n_max >> r.targets_per_lane >> r.chunk_items) || (ss >> extra))
+            raise(PV_ERR_INVALID_ARGUMENT, where + " is malformed");
+        r.geometry = geo == "infinite" ? PV_INFINITE
+                     : geo == "periodic" ? PV_PERIODIC
+                     : geo == "disk"     ? PV_DISK
+                                         : -1;
+        r.precision = prec == "fp64" ? PV_FP64 : prec == "mixed" ? PV_MIXED : -1;
+        r.n_max = (size_t)n_max;
+        if (r.geometry < 0 || r.precision < 0)
+            raise(PV_ERR_INVALID_ARGUMENT, where + " names an unknown geometry or precision");
+        rules.push_back(r);
+    }
+    return rules;
+}
+
+void check_batch_rules(const pv_batch_rule *rules, size_t count) {
+    for (size_t k = 0; k < count; ++k) {
+        const pv_batch_rule &r = rules[k];
+        if (r.geometry < PV_INFINITE || r.geometry > PV_DISK ||
+            (r.precision != PV_FP64 && r.precision != PV_MIXED))
+            raise(PV_ERR_INVALID_ARGUMENT, "launch rule names an unknown geometry or precision");
+        if (r.targets_per_lane != 0 && r.targets_per_lane != 1 && r.targets_per_lane != 2 &&
+            r.targets_per_lane != 4)
+            raise(PV_ERR_INVALID_ARGUMENT, "targets per lane must be 0, 1, 2 or 4");
+        if (r.chunk_items < 0 || r.chunk_items > 64)
+            raise(PV_ERR_INVALID_ARGUMENT, "chunk items must be between 0 and 64");
+    }
+}
+
+// The default table of a new ensemble: $PV_HIP_ENSEMBLE_TABLE, else ensemble_table.txt beside
+// this library when present, else none (2 targets per lane, chunks of 4).
+std::vector<pv_batch_rule> default_batch_table() {
+    std::string path;
+    if (const char *env = std::getenv("PV_HIP_ENSEMBLE_TABLE"); env && *env) {
+        path = env;
+    } else {
+        Dl_info info{};
+        if (!dladdr((void *)&pv_batch_create, &info) || !info.dli_fname)
+            return {};
+        path = info.dli_fname;
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+               "/ensemble_table.txt";
+        if (!std::ifstream(path))
+            return {};
+    }
+    auto rules = read_batch_table(path);
+    check_batch_rules(rules.data(), rules.size());
+    return rules;
+}
 
 // src/timestep.h:8-15 and src/timestep.cpp:69-73.
 constexpr double kB5[7] = {35.0 / 384.0,     0.0, 500.0 / 1113.0, 125.0 / 192.0,
@@ -50,13 +133,14 @@ pv_config to_config(const pv_instance_config &c, int precision) {
     return p;
 }
 
-// Relative cost of one pair by geometry (measured kernel rates at N = 64K), used only to
-// order work items.
+// Relative cost of one pair by geometry, used only to order work items and cut chunks: the
+// ensemble's RK4 step times of four N=65536 instances at 2 targets per lane, against the
+// infinite plane's (FP64 periodic 6.3x, disk 1.75x; mixed 4.1x and 2.0x).
 double pair_weight(int geometry, int precision) {
     if (geometry == PV_PERIODIC)
-        return precision ? 4.0 : 12.0;
+        return precision ? 4.1 : 6.3;
     if (geometry == PV_DISK)
-        return precision ? 1.9 : 2.0;
+        return precision ? 2.0 : 1.75;
     return 1.0;
 }
 
@@ -88,6 +172,12 @@ struct pv_batch {
     int precision = 0;                 // of the instances loaded or admitted next
     int cu_count = 256;
     hipStream_t stream = nullptr;
+    // Symmetric groups run side by side: group g on `stream` (g = 0) or side[g - 1], forked
+    // from and joined back to `stream` by events, so that one group's tail overlaps the
+    // others' work (eager launches and captured graphs alike).
+    static constexpr int kSides = pv::kSymGroups - 1;
+    hipStream_t side[kSides] = {};
+    hipEvent_t fork = nullptr, join[kSides] = {};
     size_t capacity = 0, max_inst = 0;
     // The per-vortex arrays below hold vcap <= capacity vortices; they grow as instances are
     // loaded or admitted.  byte_limit (0: none) bounds everything the ensemble allocates on
@@ -114,8 +204,18 @@ struct pv_batch {
     bool sym = false;                  // the mode the current work items were built for
     pv::PairItem *pitems_d = nullptr;
     size_t pitems_cap = 0;
-    int *chunks_d = nullptr;           // pitems_cap + 1 chunk starts
+    int *chunks_d = nullptr;           // chunk starts of every group (chunks_cap of them)
+    size_t chunks_cap = 0;
     int nchunks = 0;
+    // The launch groups of the current symmetric items (pv::sym_group: 0 every item at its
+    // instance's width, 1 and 2 the periodic and disk 4-target items), each with its items at
+    // pitems_d + item_off and its chunk starts (relative to item_off) at chunks_d + chunk_off.
+    struct Group {
+        int group, precisions;
+        int item_off, nitems, chunk_off, nchunks;
+    };
+    std::vector<Group> groups;
+    std::vector<pv_batch_rule> table;  // tile and chunk rules of the symmetric decomposition
     double *part = nullptr;
     size_t part_cap = 0;               // doubles
     int span = 0;                      // flat vortices the reduction covers
@@ -156,6 +256,16 @@ struct pv_batch {
             (void)hipHostFree(steps_h);
         if (st_host)
             (void)hipHostFree(st_host);
+        for (int k = 0; k < kSides; ++k) {
+            if (side[k])
+                (void)hipStreamSynchronize(side[k]);
+            if (side[k])
+                (void)hipStreamDestroy(side[k]);
+            if (join[k])
+                (void)hipEventDestroy(join[k]);
+        }
+        if (fork)
+            (void)hipEventDestroy(fork);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -174,8 +284,8 @@ void reset_status(pv_batch *b) {
     hip_check(hipGetLastError(), "launch HIP status reset");
 }
 
-int batch_blocks(const pv_batch *b) {
-    const int want = (b->nitems + 3) / 4;
+int batch_blocks(const pv_batch *b, int nitems) {
+    const int want = (nitems + 3) / 4;
     return std::max(1, std::min(want, 8 * b->cu_count));
 }
 
@@ -188,6 +298,7 @@ void launch(pv_batch *b, const double *px, const double *py, const pv::StageOp &
     a.items = b->items_d;
     a.nitems = b->nitems;
     a.counter = b->counter;
+    a.next = b->counter;
     a.x = px, a.y = py, a.g = b->g;
     a.st = b->st;
     a.epoch = ++b->epoch;
@@ -195,19 +306,44 @@ void launch(pv_batch *b, const double *px, const double *py, const pv::StageOp &
     a.steps = b->steps_d;
     a.pass = pass;
     if (b->sym) {
-        a.pitems = b->pitems_d;
-        a.chunks = b->chunks_d;
-        a.nchunks = b->nchunks;
+        // One launch per group, each with its own chunk counter; they write disjoint partial
+        // slots, and the reduction after them adds every vortex's slots.
         a.part = b->part;
         a.owner = b->owner;
         a.span = b->span;
-        pv::launch_batch_sym(b->item_precisions, adaptive, a, batch_blocks(b), b->stream);
-        hip_check(hipGetLastError(), "launch HIP ensemble pair kernel");
+        const int ngroups = (int)b->groups.size();
+        const int sides = std::min(ngroups - 1, pv_batch::kSides);
+        if (sides > 0) {
+            hip_check(hipEventRecord(b->fork, b->stream), "record HIP event");
+            for (int k = 0; k < sides; ++k)
+                hip_check(hipStreamWaitEvent(b->side[k], b->fork, 0), "wait on HIP event");
+        }
+        for (int g = 0; g < ngroups; ++g) {
+            const pv_batch::Group &gr = b->groups[g];
+            const int lane = g;     // at most kSides + 1 groups
+            a.pitems = b->pitems_d + gr.item_off;
+            a.chunks = b->chunks_d + gr.chunk_off;
+            a.nchunks = gr.nchunks;
+            a.nitems = gr.nitems;
+            a.next = b->counter + 2 + gr.group;
+            // Each group has a persistent grid of its own: blocks of a group whose chunks are
+            // done exit and free their slots for the others' (measured on the 4096-instance
+            // ensemble with three groups: grids split in proportion to the groups' estimated
+            // costs were 9 % slower in FP64, the groups one after another 4 % slower mixed).
+            pv::launch_batch_sym(gr.precisions, adaptive, gr.group, a,
+                                 batch_blocks(b, gr.nitems),
+                                 lane == 0 ? b->stream : b->side[lane - 1]);
+            hip_check(hipGetLastError(), "launch HIP ensemble pair kernel");
+        }
+        for (int k = 0; k < sides; ++k) {
+            hip_check(hipEventRecord(b->join[k], b->side[k]), "record HIP event");
+            hip_check(hipStreamWaitEvent(b->stream, b->join[k], 0), "wait on HIP event");
+        }
         pv::launch_batch_sym_reduce(adaptive, a, b->stream);
         hip_check(hipGetLastError(), "launch HIP ensemble reduction");
         return;
     }
-    pv::launch_batch(b->item_precisions, adaptive, a, batch_blocks(b), b->stream);
+    pv::launch_batch(b->item_precisions, adaptive, a, batch_blocks(b, b->nitems), b->stream);
     hip_check(hipGetLastError(), "launch HIP ensemble kernel");
 }
 
@@ -215,63 +351,108 @@ bool wants_sym(const pv_batch *b) {
     return b->sym_mode == 1 || (b->sym_mode < 0 && !b->adaptive);
 }
 
+// Symmetric launch shape of an instance: targets per lane (tiles of 64 tpt vortices) and the
+// most items a wave takes per counter increment.  The first rule of the ensemble's table that
+// covers (geometry, precision, n) decides; otherwise 2 targets per lane and chunks of 4.
+struct SymShape {
+    int tpt, chunk;
+};
+constexpr int kChunkItemsDefault = 4;
+
+SymShape sym_shape(const pv_batch *b, int geometry, int precision, size_t n) {
+    for (const pv_batch_rule &r : b->table)
+        if (r.geometry == geometry && r.precision == precision && n <= r.n_max)
+            return {r.targets_per_lane ? r.targets_per_lane : pv::kSymTptDefault,
+                    r.chunk_items ? r.chunk_items : kChunkItemsDefault};
+    return {pv::kSymTptDefault, kChunkItemsDefault};
+}
+
 // Symmetric work items: every tile pair (I <= J) of every listed instance, each instance's
-// partial sums placed after the previous one's, longest items first.
+// partial sums placed after the previous one's.  Items are grouped by pv::sym_group, one
+// launch per group, longest items first within a group.
 void build_pair_items(pv_batch *b, const std::vector<int> &instances) {
     struct Keyed {
         double cost;
         pv::PairItem item;
+        int chunk;          // the instance's chunk limit
     };
-    constexpr int TT = pv::kSymTile;
-    std::vector<Keyed> v;
+    std::vector<Keyed> per[pv::kSymGroups];
+    int group_precisions[pv::kSymGroups] = {};
     size_t poff = 0;
     for (int k : instances) {
         pv::Instance &in = b->inst_h[k];
+        const SymShape sh = sym_shape(b, in.gp.geometry, in.precision, (size_t)in.n);
+        const int TT = 64 * sh.tpt;
         const int nt = (in.n + TT - 1) / TT;
+        in.tpt = sh.tpt;
         in.nt = nt;
         in.poff = poff;
         poff += (size_t)nt * 2 * (size_t)in.n;
+        const int g = pv::sym_group(sh.tpt, in.gp.geometry);
+        group_precisions[g] |= in.precision ? 2 : 1;
         const double w = pair_weight(in.gp.geometry, in.precision);
         for (int J = 0; J < nt; ++J) {
             const double rj = std::min(TT, in.n - J * TT);
             for (int I = 0; I <= J; ++I) {
                 const double ri = std::min(TT, in.n - I * TT);
-                v.push_back({w * ri * rj * (I == J ? 1.0 : 0.5), {k, I, J, 0}});
+                per[g].push_back({w * ri * rj * (I == J ? 1.0 : 0.5), {k, I, J, 0}, sh.chunk});
             }
         }
     }
-    std::stable_sort(v.begin(), v.end(),
-                     [](const Keyed &a, const Keyed &c) { return a.cost > c.cost; });
-    // Chunks of at most kChunkItems consecutive items (every item has a fixed fetch and
-    // staging latency besides its pairs) and of at most total / kChunksWanted cost (an item
-    // costlier than that is a chunk by itself).
-    constexpr int kChunkItems = 4;
+    // Within a group: chunks of at most the first item's instance limit of consecutive items
+    // (every item has a fixed fetch and staging latency besides its pairs) and of at most the
+    // group's total / kChunksWanted cost (an item costlier than that is a chunk by itself).
     constexpr double kChunksWanted = 16384.0;
-    double total_cost = 0.0;
-    for (const Keyed &k : v)
-        total_cost += k.cost;
-    const double quota = total_cost / kChunksWanted;
-    std::vector<int> chunks{0};
-    double acc = 0.0;
-    for (size_t i = 0; i < v.size(); ++i) {
-        const size_t held = i - (size_t)chunks.back();
-        if (held > 0 && (held >= (size_t)kChunkItems || acc + v[i].cost > quota)) {
-            chunks.push_back((int)i);
-            acc = 0.0;
+    std::vector<pv::PairItem> items;
+    std::vector<int> chunks;
+    std::vector<pv_batch::Group> groups;
+    for (int g = 0; g < pv::kSymGroups; ++g) {
+        std::vector<Keyed> &v = per[g];
+        if (v.empty())
+            continue;
+        std::stable_sort(v.begin(), v.end(),
+                         [](const Keyed &x, const Keyed &y) { return x.cost > y.cost; });
+        double total_cost = 0.0;
+        for (const Keyed &k : v)
+            total_cost += k.cost;
+        const double quota = total_cost / kChunksWanted;
+        pv_batch::Group gr{};
+        gr.group = g;
+        gr.precisions = group_precisions[g];
+        gr.item_off = (int)items.size();
+        gr.nitems = (int)v.size();
+        gr.chunk_off = (int)chunks.size();
+        size_t start = 0;
+        double acc = 0.0;
+        chunks.push_back(0);
+        for (size_t i = 0; i < v.size(); ++i) {
+            const size_t held = i - start;
+            if (held > 0 && (held >= (size_t)v[start].chunk || acc + v[i].cost > quota)) {
+                chunks.push_back((int)i);
+                start = i;
+                acc = 0.0;
+            }
+            acc += v[i].cost;
         }
-        acc += v[i].cost;
-    }
-    if (!v.empty())
         chunks.push_back((int)v.size());
-    if (v.size() > b->pitems_cap || poff > b->part_cap) {
+        gr.nchunks = (int)chunks.size() - gr.chunk_off - 1;
+        for (const Keyed &k : v)
+            items.push_back(k.item);
+        groups.push_back(gr);
+    }
+    if (items.size() > b->pitems_cap || chunks.size() > b->chunks_cap || poff > b->part_cap) {
         hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
-        if (v.size() > b->pitems_cap) {
+        if (items.size() > b->pitems_cap) {
             dfree(b->pitems_d);
-            dfree(b->chunks_d);
             b->pitems_cap = 0;
-            b->pitems_d = dalloc<pv::PairItem>(v.size(), "hipMalloc(work items)");
-            b->chunks_d = dalloc<int>(v.size() + 1, "hipMalloc(work chunks)");
-            b->pitems_cap = v.size();
+            b->pitems_d = dalloc<pv::PairItem>(items.size(), "hipMalloc(work items)");
+            b->pitems_cap = items.size();
+        }
+        if (chunks.size() > b->chunks_cap) {
+            dfree(b->chunks_d);
+            b->chunks_cap = 0;
+            b->chunks_d = dalloc<int>(chunks.size(), "hipMalloc(work chunks)");
+            b->chunks_cap = chunks.size();
         }
         if (poff > b->part_cap) {
             dfree(b->part);
@@ -280,9 +461,6 @@ void build_pair_items(pv_batch *b, const std::vector<int> &instances) {
             b->part_cap = poff;
         }
     }
-    std::vector<pv::PairItem> items(v.size());
-    for (size_t i = 0; i < v.size(); ++i)
-        items[i] = v[i].item;
     // The instance table is host-authoritative: the device never writes it.
     hip_check(hipMemcpyAsync(b->inst_d, b->inst_h.data(), b->max_inst * sizeof(pv::Instance),
                              hipMemcpyHostToDevice, b->stream),
@@ -297,7 +475,8 @@ void build_pair_items(pv_batch *b, const std::vector<int> &instances) {
     }
     hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
     b->nitems = (int)items.size();
-    b->nchunks = items.empty() ? 0 : (int)chunks.size() - 1;
+    b->nchunks = (int)(chunks.size() - groups.size());
+    b->groups = std::move(groups);
 }
 
 void build_items(pv_batch *b, const std::vector<int> &instances);
@@ -421,7 +600,7 @@ void clear_instances(pv_batch *b) {
     b->items_dirty = true;
     b->item_instances.clear();
     b->steps_done = 0;
-    hip_check(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), b->stream),
+    hip_check(hipMemsetAsync(b->counter, 0, kCounterWords * sizeof(unsigned), b->stream),
               "clear step count");
 }
 
@@ -487,21 +666,24 @@ constexpr size_t kVortexBytes = kDoublesPerVortex * sizeof(double) + sizeof(int)
 // Device bytes that do not depend on the vortices: instance tables, counters, status.
 size_t fixed_bytes(const pv_batch *b) {
     return b->max_inst * (sizeof(pv::Instance) + sizeof(pv::InstanceStep)) +
-           2 * sizeof(unsigned) + sizeof(pv::DevStatus);
+           kCounterWords * sizeof(unsigned) + sizeof(pv::DevStatus);
 }
 
 // Everything the ensemble holds on the device now.
 size_t device_bytes(const pv_batch *b) {
     return fixed_bytes(b) + b->vcap * kVortexBytes + b->part_cap * sizeof(double) +
-           b->pitems_cap * (sizeof(pv::PairItem) + sizeof(int)) +
-           (b->pitems_cap ? sizeof(int) : 0) + b->items_cap * sizeof(pv::WorkItem);
+           b->pitems_cap * sizeof(pv::PairItem) + b->chunks_cap * sizeof(int) +
+           b->items_cap * sizeof(pv::WorkItem);
 }
 
-// Work-item and partial-sum bytes one running instance of n vortices needs in `sym` mode.
-size_t instance_bytes(size_t n, bool sym) {
+// Work-item and partial-sum bytes one running instance of n vortices needs in `sym` mode (its
+// tile from the ensemble's launch rules).  The chunk starts are counted as one per item plus
+// one, which bounds what the groups need.
+size_t instance_bytes(const pv_batch *b, int geometry, int precision, size_t n, bool sym) {
     if (!sym)
         return (n + 63) / 64 * sizeof(pv::WorkItem);
-    const size_t nt = (n + pv::kSymTile - 1) / pv::kSymTile;
+    const size_t tile = 64 * (size_t)sym_shape(b, geometry, precision, n).tpt;
+    const size_t nt = (n + tile - 1) / tile;
     return nt * (nt + 1) / 2 * (sizeof(pv::PairItem) + sizeof(int)) + sizeof(int) +
            nt * 2 * n * sizeof(double);
 }
@@ -547,8 +729,9 @@ void grow_vortices(pv_batch *b, size_t newcap) {
 void trim_items(pv_batch *b) {
     hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
     dfree(b->part), dfree(b->pitems_d), dfree(b->chunks_d), dfree(b->items_d);
-    b->part_cap = b->pitems_cap = b->items_cap = 0;
+    b->part_cap = b->pitems_cap = b->chunks_cap = b->items_cap = 0;
     b->nitems = b->nchunks = 0;
+    b->groups.clear();
     b->items_dirty = true;
 }
 
@@ -574,9 +757,48 @@ pv_status pv_batch_device_bytes(pv_batch *b, size_t *bytes) {
     });
 }
 
-size_t pv_batch_symmetric_bytes(size_t n) {
-    const size_t tiles = (n + pv::kSymTile - 1) / pv::kSymTile;
-    return tiles * 2 * n * sizeof(double);
+pv_status pv_batch_instance_bytes(pv_batch *b, int geometry, int precision, size_t n, int sym,
+                                  size_t *bytes) {
+    return guard([&] {
+        checked(b);
+        if (!bytes)
+            raise(PV_ERR_INVALID_ARGUMENT, "null output");
+        if (geometry < PV_INFINITE || geometry > PV_DISK ||
+            (precision != PV_FP64 && precision != PV_MIXED))
+            raise(PV_ERR_INVALID_ARGUMENT, "unknown geometry or precision");
+        *bytes = instance_bytes(b, geometry, precision, n, sym != 0);
+    });
+}
+
+pv_status pv_batch_set_launch_table(pv_batch *b, const pv_batch_rule *rules, size_t count) {
+    return guard([&] {
+        checked(b);
+        if (count && !rules)
+            raise(PV_ERR_INVALID_ARGUMENT, "null argument");
+        check_batch_rules(rules, count);
+        DeviceGuard dg(b->device);
+        hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
+        b->table.assign(rules, rules + count);
+        for (auto &kv : b->graphs)
+            (void)hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        if (b->adaptive)
+            b->items_dirty = true;
+        else if (!b->item_instances.empty())
+            build_items(b, std::vector<int>(b->item_instances));
+    });
+}
+
+pv_status pv_batch_resolve_launch(pv_batch *b, int geometry, int precision, size_t n,
+                                  int *targets_per_lane, int *chunk_items) {
+    return guard([&] {
+        checked(b);
+        if (!targets_per_lane || !chunk_items)
+            raise(PV_ERR_INVALID_ARGUMENT, "null argument");
+        const SymShape sh = sym_shape(b, geometry, precision, n);
+        *targets_per_lane = sh.tpt;
+        *chunk_items = sh.chunk;
+    });
 }
 
 pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max_instances,
@@ -603,7 +825,15 @@ pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max
         hipDeviceProp_t prop{};
         hip_check(hipGetDeviceProperties(&prop, device), "query HIP device properties");
         b->cu_count = std::max(1, prop.multiProcessorCount);
+        b->table = default_batch_table();
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "create HIP stream");
+        for (int k = 0; k < pv_batch::kSides; ++k) {
+            hip_check(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking),
+                      "create HIP stream");
+            hip_check(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming),
+                      "create HIP event");
+        }
+        hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "create HIP event");
         b->inst_d = dalloc<pv::Instance>(max_instances, "hipMalloc(instances)");
         b->steps_d = dalloc<pv::InstanceStep>(max_instances, "hipMalloc(instance states)");
         hip_check(hipHostMalloc((void **)&b->steps_h, max_instances * sizeof(pv::InstanceStep),
@@ -619,8 +849,8 @@ pv_status pv_batch_create(int device, int precision, size_t capacity, size_t max
                                  hipMemcpyHostToDevice, b->stream),
                   "upload instance states");
         hip_check(hipStreamSynchronize(b->stream), "synchronize HIP stream");
-        b->counter = dalloc<unsigned>(2, "hipMalloc(work counter)");
-        hip_check(hipMemsetAsync(b->counter, 0, 2 * sizeof(unsigned), b->stream),
+        b->counter = dalloc<unsigned>(kCounterWords, "hipMalloc(work counter)");
+        hip_check(hipMemsetAsync(b->counter, 0, kCounterWords * sizeof(unsigned), b->stream),
                   "clear work counter");
         b->st = dalloc<pv::DevStatus>(1, "hipMalloc(status)");
         hip_check(hipHostMalloc((void **)&b->st_host, sizeof(pv::DevStatus), hipHostMallocDefault),
@@ -701,7 +931,8 @@ pv_status pv_batch_load(pv_batch *b, const pv_instance_config *configs, const si
         if (b->byte_limit) {
             size_t need = fixed_bytes(b) + std::max(total, b->vcap) * kVortexBytes;
             for (size_t k = 0; k < ninst; ++k)
-                need += instance_bytes(counts[k], b->sym_mode != 0);
+                need += instance_bytes(b, configs[k].geometry, b->precision, counts[k],
+                                       b->sym_mode != 0);
             if (need > b->byte_limit)
                 raise(PV_ERR_RUNTIME, "ensemble would exceed its device byte limit");
         }
@@ -926,10 +1157,11 @@ pv_status pv_batch_admit(pv_batch *b, const pv_instance_config *config, const do
                               std::max({need, b->vcap + b->vcap / 2, (size_t)1 << 16}));
         if (b->byte_limit) {
             const bool sym = b->sym_mode == 1;
-            size_t items = instance_bytes(n, sym);
+            size_t items = instance_bytes(b, config->geometry, b->precision, n, sym);
             for (size_t q = 0; q < b->max_inst; ++q)
                 if (b->used[q])
-                    items += instance_bytes((size_t)b->inst_h[q].n, sym);
+                    items += instance_bytes(b, b->inst_h[q].gp.geometry, b->inst_h[q].precision,
+                                            (size_t)b->inst_h[q].n, sym);
             const size_t held = device_bytes(b) - fixed_bytes(b) - b->vcap * kVortexBytes;
             auto projected = [&](size_t cap, size_t buffers) {
                 const size_t growing = cap > b->vcap ? b->vcap * sizeof(double) : 0;

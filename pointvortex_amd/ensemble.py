@@ -15,8 +15,8 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .backend import (GEOMETRIES, PRECISIONS, PvInstanceConfig, PvSlotState, _check, _f64,
-                      _ptr, _szp, _vp, load_library)
+from .backend import (GEOMETRIES, PRECISIONS, PvBatchRule, PvInstanceConfig, PvSlotState,
+                      _check, _f64, _ptr, _szp, _vp, load_library)
 
 STATES = {0: "running", 1: "finished", 2: "failed", 3: "idle"}
 
@@ -52,11 +52,6 @@ class SlotState:
 
 def bytes_per_vortex() -> int:
     return int(load_library().pv_batch_bytes_per_vortex())
-
-
-def symmetric_bytes(n: int) -> int:
-    """Partial-sum bytes of one n-vortex instance in the symmetric decomposition."""
-    return int(load_library().pv_batch_symmetric_bytes(int(n)))
 
 
 class Ensemble:
@@ -105,6 +100,33 @@ class Ensemble:
     @property
     def symmetric(self) -> bool:
         return bool(self.lib.pv_batch_symmetric(self._h))
+
+    def set_launch_table(self, rules):
+        """rules: iterable of (geometry, precision, n_max, targets_per_lane, chunk_items) for
+        the symmetric decomposition (tiles of 64 targets_per_lane vortices); [] restores the
+        defaults (2 targets per lane, chunks of 4)."""
+        rules = list(rules)
+        arr = (PvBatchRule * max(1, len(rules)))(*[
+            PvBatchRule(GEOMETRIES[g], PRECISIONS[p], int(n), int(t), int(c))
+            for g, p, n, t, c in rules])
+        _check(self.lib.pv_batch_set_launch_table(self._h, arr, len(rules)))
+
+    def resolve_launch(self, geometry: str, precision: str, n: int):
+        """(targets_per_lane, chunk_items) an instance of n vortices takes."""
+        t, c = ctypes.c_int(0), ctypes.c_int(0)
+        _check(self.lib.pv_batch_resolve_launch(self._h, GEOMETRIES[geometry],
+                                                PRECISIONS[precision], int(n), ctypes.byref(t),
+                                                ctypes.byref(c)))
+        return t.value, c.value
+
+    def instance_bytes(self, geometry: str, precision: str, n: int, symmetric: bool = True):
+        """Device bytes one admitted run of n vortices adds beyond its per-vortex arrays (work
+        items, and in the symmetric decomposition its partial sums at its tile)."""
+        out = ctypes.c_size_t(0)
+        _check(self.lib.pv_batch_instance_bytes(self._h, GEOMETRIES[geometry],
+                                                PRECISIONS[precision], int(n),
+                                                int(bool(symmetric)), ctypes.byref(out)))
+        return out.value
 
     # -- fixed-step ensembles
     def load(self, configs: Sequence[InstanceConfig], xs, ys, gs):

@@ -5,14 +5,14 @@ attempts the loop admits, in arrival order, every arrived request that fits the 
 new runs join runs already in flight; finished or failed runs leave and free their bytes.  The
 budget is the ensemble's device byte limit: a run is admitted only if everything the ensemble
 then holds on the device -- its per-vortex state (208 bytes per vortex), the work items and
-partial sums of every admitted run (about 16 bytes per vortex per 128 vortices of the run), and
-its tables -- stays within it (`Ensemble.set_byte_limit`).  Each run keeps its state, FSAL
+partial sums of every admitted run (16 bytes per vortex per tile of the run: 64, 128 or 256
+vortices by its launch rule), and its tables -- stays within it (`Ensemble.set_byte_limit`).  Each run keeps its state, FSAL
 derivative and step-size controller on the device; one attempt advances every running run by
 one DOPRI5 attempt (seven stage evaluations).  Between polls the loop runs up to `max_batch`
 attempts back to back with one synchronisation: one while an arrived request waits for budget,
 never past the next arrival, and never more than a quarter of the attempts the youngest running
 run has taken, so that a run finishes at most about a quarter of its attempts before the loop
-sees it.  Runs are split into pairs of 128-vortex tiles, each unordered pair computed once
+sees it.  Runs are split into pairs of tiles, each unordered pair computed once
 (`Ensemble.set_symmetric`), so a lone large run still spreads over the GPU.  Latency is
 measured from arrival to completion.
 
@@ -30,7 +30,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import workloads as wl
-from .ensemble import Ensemble, InstanceConfig, bytes_per_vortex, symmetric_bytes
+from .ensemble import Ensemble, InstanceConfig, bytes_per_vortex
 
 
 @dataclass
@@ -112,9 +112,14 @@ class Server:
         self.max_batch = max(1, int(max_batch))
         self._engine: Optional[Ensemble] = None
 
-    def footprint(self, n: int) -> int:
-        """HBM bytes a run of n vortices holds while admitted (state and partial sums)."""
-        return n * self.per_vortex + (symmetric_bytes(n) if self.symmetric else 0)
+    def footprint(self, n: int, geometry: str = "disk", precision: str = "fp64") -> int:
+        """HBM bytes a run of n vortices holds while admitted: its state, and its work items
+        and partial sums (at the tile the ensemble's launch rules give it)."""
+        return n * self.per_vortex + self.engine.instance_bytes(geometry, precision, n,
+                                                                 self.symmetric)
+
+    def request_footprint(self, req: Request) -> int:
+        return self.footprint(req.n, req.geometry, self.precision_for(req))
 
     def precision_for(self, req: Request) -> str:
         return "mixed" if req.rtol >= self.threshold else "fp64"
@@ -159,7 +164,7 @@ class Server:
                         raise RuntimeError(f"request {req.rid} (N={req.n}) exceeds the budget")
                     break
                 self._track()
-                self.used += self.footprint(req.n)
+                self.used += self.request_footprint(req)
                 self.peak_used = max(self.peak_used, self.used)
                 active[slot] = (req, clock() - t0, prec)
                 pending.popleft()
@@ -195,7 +200,7 @@ class Server:
                 if self.keep and s.state == "finished":
                     x, y = eng.fetch(s.slot, s.n)
                 eng.release(s.slot)
-                self.used -= self.footprint(req.n)
+                self.used -= self.request_footprint(req)
                 stats.results.append(Result(req.rid, s.state, req.arrival, admitted, now,
                                             s.accepted, s.rejected, s.failure, prec, x, y))
                 if on_result is not None:

@@ -196,8 +196,10 @@ def test_c4_failing_disk_instances_fail_in_the_oracle_too(oracle, c4):
 @pytest.mark.timeout(1200)
 def test_c5_serving_admits_large_runs_under_a_binding_budget(oracle):
     """The serving loop with runs at N=131072 and N=65536 (disk populations of the benchmark's
-    kind, FP64 at rtol 1e-8) beside small runs of both precisions, under a 2.5 GB budget that
-    cannot hold both large runs at once: the second waits until the first has left.  The
+    kind, FP64 at rtol 1e-8) beside small runs of both precisions, under a budget that cannot
+    hold both large runs at once (the larger run's footprint plus half the smaller one's, about
+    1.2-2.4 GB by the tile the launch rules give them): the second waits until the first has
+    left.  The
     device allocation never exceeds the budget; each large run gives the same bits as when
     served alone; and the final state's FP64 velocities agree with the oracle on the first 256
     targets and the wall shell (the last 256)."""
@@ -211,7 +213,8 @@ def test_c5_serving_admits_large_runs_under_a_binding_budget(oracle):
                      80 + k, 2e-4, 1e-8 if k % 2 else 1e-5, 1e-11 if k % 2 else 1e-8)
              for k in range(10)]
     reqs = big + small
-    budget = int(2.5e9)
+    probe = Server(hbm_budget_bytes=int(8e9))
+    budget = probe.footprint(131072) + probe.footprint(65536) // 2
     srv = Server(hbm_budget_bytes=budget, max_runs=16, keep_results=True)
     assert srv.footprint(131072) + srv.footprint(65536) > budget
     stats = srv.run(reqs, max_wall=600.0)

@@ -477,13 +477,13 @@ def test_serving_budget_counts_both_precisions_and_partials():
     reqs = make_requests(6, 9, rate=0.0, n_min=500, n_max=700, t_min=2e-4, t_max=4e-4,
                          rtol_min=1e-8, rtol_max=1e-5)
     probe = Server(hbm_budget_bytes=1 << 30)
-    big = max(probe.footprint(r.n) for r in reqs)
+    big = max(probe.request_footprint(r) for r in reqs)
     assert big > 700 * bytes_per_vortex()          # the partials are counted
     srv = Server(hbm_budget_bytes=int(2.5 * big), max_runs=8)
     stats = srv.run(reqs, max_wall=60.0)
     assert stats.summary()["completed"] == len(reqs)
     assert srv.peak_used <= srv.budget
-    assert srv.peak_used >= 2 * min(srv.footprint(r.n) for r in reqs)   # runs overlapped
+    assert srv.peak_used >= 2 * min(srv.request_footprint(r) for r in reqs)   # overlapped
     assert srv.used == 0
     # What the one ensemble actually held on the device, both precisions in it at once.
     assert {r.precision for r in stats.results} == {"fp64", "mixed"}

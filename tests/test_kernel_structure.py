@@ -95,8 +95,9 @@ def _loop_of_each_line(lines):
 
 def test_every_persistent_kernel_was_found(batch_asm):
     names = [n for n, _ in _kernels(batch_asm)]
-    # {ordered, symmetric} x {fp64, mixed, both in one launch} x {fixed, adaptive}
-    assert len(names) == 12, names
+    # ordered: {fp64, mixed, both in one launch} x {fixed, adaptive}; symmetric: the same
+    # x {per-instance widths, periodic 4-target items, disk 4-target items}
+    assert len(names) == 6 + 6 * 3, names
 
 
 def test_item_fetch_and_item_load_share_the_outermost_loop(batch_asm):
