@@ -13,9 +13,14 @@ the FP64 disk evaluation's algorithmic FLOP rate (36 flops per ordered pair, N(N
 over its average duration from HIP events on the handle's stream.  With the tuned launch table
 an evaluation at this N is the symmetric path -- the pair kernel over unordered tile pairs
 (each pair computed once, both sides) and its reduction kernel, both inside the events -- so
-the hardware executes about half the algorithmic pair count.  `cpu_baseline` is the reference's own CPU
-kernel (oracle/_ref, built from the reference sources) or, if absent, the repo's C
-restatement, timed on this host's cores on a bounded sample.
+the hardware executes about half the algorithmic pair count.  Before its own GPU work the
+benchmark runs three rocprofv3 counter passes over the same evaluation in child processes
+(pointvortex_amd/profiling.py): `roofline.traffic` is the HBM bytes of one evaluation
+(FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated) and `roofline.fp64_issue` the share of the
+pair kernel's SIMD cycles spent issuing FP64 instructions (--no-counters skips them).
+`cpu_baseline` is the reference's own CPU kernel (oracle/_ref, built from the reference
+sources) or, if absent, the repo's C restatement, timed on this host's cores on a bounded
+sample.
 """
 import argparse
 import json
@@ -34,20 +39,6 @@ FLOPS_PER_PAIR = {"infinite": 12, "disk": 36, "periodic": 12 + 22 * 17}
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 
-
-TRAFFIC_SUMMARY = ROOT / "profiles" / "sym_kernel_disk_f64_n65536_hbm_traffic.md"
-
-
-def measured_traffic():
-    """HBM bytes per launch of the FP64 disk pair kernel at N=65536 from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), or None when absent."""
-    try:
-        for line in TRAFFIC_SUMMARY.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":", 1)[1])
-    except OSError:
-        pass
-    return None
 
 
 def log(*a):
@@ -368,33 +359,62 @@ def cpu_samples(threads):
     return out
 
 
-def serving_bench(pv, count=128, seed=5):
-    """Continuous-batching request loop, scaled down so that it runs in about a minute:
-    N log-uniform in [64, 2048], horizon U[1e-4, 1e-3], rtol log-uniform in [1e-10, 1e-6],
-    atol = 1e-3 rtol, geometry uniform, 4 GB HBM budget.  Capacity is measured first with every
-    request arriving at t=0; the measured run then uses Poisson arrivals at 80 % of it."""
+SERVING = dict(n_min=64, n_max=2048, t_min=1e-4, t_max=1e-3, rtol_min=1e-10, rtol_max=1e-6)
+SERVING_WIDE = dict(n_min=64, n_max=131072, t_min=1e-8, t_max=5e-8, rtol_min=1e-10,
+                    rtol_max=1e-6, dt0=1e-10)
+
+
+def serving_bench(pv, count=128, seed=5, kw=SERVING, budget=4 << 30, max_wall=120.0,
+                  name="serving", every=False, top=()):
+    """Continuous-batching request loop, scaled down so that it runs in about a minute.
+    Default: N log-uniform in [64, 2048], horizon U[1e-4, 1e-3], rtol log-uniform in
+    [1e-10, 1e-6], atol = 1e-3 rtol, geometry uniform, 4 GB HBM budget.  The wide variant
+    (SERVING_WIDE; its last two requests set to an infinite-plane N=131072 and a disk N=65536
+    run, the top of the range the draws may miss) keeps the spec's N range [64, 131072] and its
+    256 GB budget with horizons U[1e-8, 5e-8] from dt = 1e-10: a +-1 population of 131072
+    vortices has pairs about 1.5e-5 apart, turning at about 7e8 rad/s, so rtol 1e-10 holds
+    DOPRI5 to steps near 1e-11 (a few hundred attempts for the largest run).  Capacity is measured first with every request arriving at t=0; the measured run
+    then uses Poisson arrivals at 80 % of it."""
     from pointvortex_amd.serve import Server, make_requests
 
     def progress(phase):
         def cb(result, stats):
-            if len(stats.results) % 16 == 0 or len(stats.results) == count:
-                log(f"[bench] serving {phase}: {len(stats.results)}/{count} done")
+            done = len(stats.results)
+            if every or done % 16 == 0 or done == count:
+                log(f"[bench] {name} {phase}: {done}/{count} done (request {result.rid}, "
+                    f"{result.state}, {result.accepted + result.rejected} attempts)")
         return cb
 
-    kw = dict(n_min=64, n_max=2048, t_min=1e-4, t_max=1e-3, rtol_min=1e-10, rtol_max=1e-6)
-    reqs = make_requests(count, seed, rate=0.0, **kw)
-    cap = Server(hbm_budget_bytes=4 << 30, max_runs=count).run(
-        reqs, max_wall=120.0, on_result=progress("capacity")).summary()
+    def requests(rate):
+        reqs = make_requests(count, seed, rate=rate, **kw)
+        for r, (n, geometry) in zip(reqs[len(reqs) - len(top):], top):   # top of the range
+            r.n, r.geometry = n, geometry
+        return reqs
+
+    reqs = requests(0.0)
+    cap = Server(hbm_budget_bytes=budget, max_runs=count).run(
+        reqs, max_wall=max_wall, on_result=progress("capacity")).summary()
     rate = 0.8 * cap["solves_per_s"]
-    reqs = make_requests(count, seed, rate=rate, **kw)
-    run = Server(hbm_budget_bytes=4 << 30, max_runs=count).run(
-        reqs, max_wall=120.0, on_result=progress("open loop")).summary()
-    log(f"[bench] serving: capacity {cap['solves_per_s']:.1f} solves/s; at {rate:.1f}/s: "
+    reqs = requests(rate)
+    srv = Server(hbm_budget_bytes=budget, max_runs=count)
+    run = srv.run(reqs, max_wall=max_wall, on_result=progress("open loop")).summary()
+    log(f"[bench] {name}: capacity {cap['solves_per_s']:.1f} solves/s; at {rate:.1f}/s: "
         f"p50 {run['p50_s']:.3f} s p99 {run['p99_s']:.3f} s")
+    ns = [r.n for r in reqs]
     return {"requests": count, "capacity_solves_per_s": cap["solves_per_s"],
-            "arrival_rate_per_s": rate, **run,
-            "workload": "N log-uniform [64,2048], t U[1e-4,1e-3], rtol log-uniform "
-                        "[1e-10,1e-6], atol 1e-3 rtol, geometry uniform, seed 5"}
+            "arrival_rate_per_s": rate, **run, "hbm_budget_bytes": budget,
+            "peak_device_bytes": srv.peak_device_bytes, "n_max_drawn": max(ns),
+            "requests_n_ge_16384": sum(n >= 16384 for n in ns),
+            "workload": (f"N log-uniform [{kw['n_min']},{kw['n_max']}], t U[{kw['t_min']:g},"
+                         f"{kw['t_max']:g}], dt0 {kw.get('dt0', 1e-3):g}, rtol log-uniform "
+                         f"[{kw['rtol_min']:g},{kw['rtol_max']:g}], atol 1e-3 rtol, geometry "
+                         f"uniform, seed {seed}")}
+
+
+def serving_wide_bench(pv):
+    return serving_bench(pv, count=64, seed=7, kw=SERVING_WIDE, budget=256 << 30,
+                         max_wall=90.0, name="serving wide", every=True,
+                         top=((131072, "infinite"), (65536, "disk")))
 
 
 def cpu_baseline(pop, budget_s=20.0):
@@ -433,10 +453,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (traffic and FP64 issue null)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the per-geometry table, the ensemble and the serving loop")
     ap.add_argument("--ensemble-steps", type=int, default=500)
-    ap.add_argument("--only", choices=["fp64", "geometries", "copy", "ensemble", "serving", "c2",
+    ap.add_argument("--only", choices=["fp64", "geometries", "copy", "ensemble", "serving",
+                                       "serving-wide", "counters", "c2",
                                        "c1"],
                     default=None,
                     help="profiling runs: only the FP64 RK4 run, only the per-geometry table, "
@@ -447,11 +470,30 @@ def main():
               "(each GPU would run its own share; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    dt = 1e-4
+    n = args.n
+    counters = None
+    if args.only is None and n == 65536 and not args.no_counters:
+        # Before this process touches the GPU: rocprofv3 counter passes over the same
+        # evaluation, in child processes (HBM traffic, FP64 issue share).
+        from pointvortex_amd.profiling import evaluation_counters
+
+        log("[bench] counter passes (rocprofv3, child processes) ...")
+        counters = evaluation_counters(n, log=log)
+
     import pointvortex_amd as pv
     from pointvortex_amd import workloads as wl
 
-    dt = 1e-4
-    n = args.n
+    if args.only == "counters":
+        # One counter pass's child: the calibration copy, then one RK4 step (four evaluations)
+        # of the FP64 disk benchmark population.
+        pv.backend.calibration_copy(256 << 20, 1)
+        k = pv.HipKernel(pv.SimParams(boundaryCondition="disk", diskRadius=1.0))
+        k.uploadDeviceState(pv.VortexSystem(*(lambda p: (p.x, p.y, p.circulation))(wl.disk(n, 2))))
+        k.deviceRk4Advance(dt, 1)
+        k.synchronize()
+        k.close()
+        return
     if args.only == "copy":
         ms = pv.backend.calibration_copy(256 << 20, 20)
         log(f"[bench] copy kernel 256 MiB: {ms:.4f} ms ({2 * (256 << 20) / ms / 1e6:.1f} GB/s)")
@@ -464,6 +506,9 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps(serving_bench(pv)), flush=True)
+        return
+    if args.only == "serving-wide":
+        print(json.dumps(serving_wide_bench(pv)), flush=True)
         return
     if args.only == "c2":
         print(json.dumps(c2_slice(pv)), flush=True)
@@ -532,8 +577,15 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": measured_traffic() if n == 65536 else None,
-            "traffic_source": str(TRAFFIC_SUMMARY.relative_to(ROOT)),
+            "traffic": counters["traffic_bytes"] if counters else None,
+            "traffic_source": ("rocprofv3 FETCH_SIZE and WRITE_SIZE passes run by this benchmark "
+                               "over one RK4 step (pair kernel + reduction per evaluation), each "
+                               "scaled by its reading of a 256 MiB copy kernel"),
+            "fp64_issue": counters["fp64_issue"] if counters else None,
+            "fp64_issue_source": ("rocprofv3 pass of SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 and "
+                                  "GRBM_GUI_ACTIVE over the pair kernel: (4 (FMA+MUL+ADD) + 16 "
+                                  "TRANS) wave cycles / (4 SIMDs x 256 CUs x cycles)"),
+            "counters": counters,
             "kernel": kernels,
             "launch_shape": {"targets_per_thread": tpt, "source_splits": splits, "tile": tile,
                              "symmetric": sym},
@@ -561,6 +613,7 @@ def main():
         result["periodic_n16384_dopri5_slice"] = c2_slice(pv)
         result["ensemble_4096"] = ensemble_bench(pv, args.ensemble_steps)
         result["serving"] = serving_bench(pv)
+        result["serving_wide"] = serving_wide_bench(pv)
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(pop)

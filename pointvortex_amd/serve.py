@@ -99,7 +99,9 @@ class Server:
                  symmetric: bool = True, max_batch: int = 16):
         self.per_vortex = bytes_per_vortex()
         self.budget = int(hbm_budget_bytes)
-        self.capacity = max(1, self.budget // self.per_vortex)
+        # The ensemble's vortex index is a 32-bit int; its arrays grow as runs are admitted,
+        # so the capacity is only a bound (the byte limit binds first on any real budget).
+        self.capacity = max(1, min(self.budget // self.per_vortex, (1 << 30) - 1))
         self.used = 0                      # footprint of the admitted runs (estimate)
         self.peak_used = 0
         self.peak_device_bytes = 0         # what the ensemble held on the device, at most
@@ -211,11 +213,13 @@ class Server:
 
 def make_requests(count: int, seed: int, rate: float, n_min: int = 64, n_max: int = 131072,
                   t_min: float = 0.1, t_max: float = 1.0, rtol_min: float = 1e-10,
-                  rtol_max: float = 1e-6, atol_factor: float = 1e-3) -> List[Request]:
+                  rtol_max: float = 1e-6, atol_factor: float = 1e-3,
+                  dt0: float = 1e-3) -> List[Request]:
     """Open-loop Poisson arrivals at `rate` per second (all at t=0 when rate is 0); geometry
     uniform over the three; N log-uniform in [n_min, n_max] (even); rtol log-uniform; horizon
-    uniform; atol = atol_factor * rtol; each request seeded by its index.  Arrival times come
-    from their own random stream, so the requests themselves do not depend on the rate."""
+    uniform; atol = atol_factor * rtol; initial step dt0; each request seeded by its index.
+    Arrival times come from their own random stream, so the requests themselves do not depend
+    on the rate."""
     rng = np.random.default_rng(seed)
     gaps = np.random.default_rng([seed, 1]).standard_exponential(count)
     t = 0.0
@@ -227,5 +231,6 @@ def make_requests(count: int, seed: int, rate: float, n_min: int = 64, n_max: in
         n = max(2, n - n % 2)
         rtol = float(np.exp(rng.uniform(np.log(rtol_min), np.log(rtol_max))))
         horizon = float(rng.uniform(t_min, t_max))
-        out.append(Request(k, t, geo, n, seed * 1_000_003 + k, horizon, rtol, atol_factor * rtol))
+        out.append(Request(k, t, geo, n, seed * 1_000_003 + k, horizon, rtol, atol_factor * rtol,
+                           dt0))
     return out

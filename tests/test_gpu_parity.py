@@ -45,7 +45,7 @@ def population(geometry, n, seed):
 # ------------------------------------------------------------------ known answers
 
 def test_single_and_pair_known_answers():
-    """src/tests.cpp:26-41: single vortex at rest; co-rotating pair v = -+1/(4 pi)."""
+    """tests/tests.cpp:26-41: single vortex at rest; co-rotating pair v = -+1/(4 pi)."""
     k = kernel_for("infinite")
     vel = k.evaluate(VortexSystem([2.0], [-3.0], [4.0]))
     assert vel.x[0] == 0.0 and vel.y[0] == 0.0
@@ -55,7 +55,7 @@ def test_single_and_pair_known_answers():
 
 
 def test_periodic_translation_invariance():
-    """src/tests.cpp:126-140 (L=2, 10 layers)."""
+    """tests/tests.cpp:126-140 (L=2, 10 layers)."""
     k = kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0, periodicImageLayers=10)
     s = VortexSystem([-0.4, 0.4], [0.1, -0.1], [1.0, -1.0])
     a = k.evaluate(s)
@@ -65,7 +65,7 @@ def test_periodic_translation_invariance():
 
 
 def test_disk_single_vortex_image():
-    """src/tests.cpp:141-153: r = 0.5, R = 2: u = 0, v = r / (2 pi (R^2 - r^2))."""
+    """tests/tests.cpp:141-153: r = 0.5, R = 2: u = 0, v = r / (2 pi (R^2 - r^2))."""
     k = kernel_for("disk", diskRadius=2.0)
     vel = k.evaluate(VortexSystem([0.5], [0.0], [1.0]))
     assert abs(vel.x[0]) <= 1e-15
@@ -73,7 +73,7 @@ def test_disk_single_vortex_image():
 
 
 def test_close_encounters():
-    """src/audit_tests.cpp:27-47."""
+    """tests/audit_tests.cpp:27-47."""
     inv2pi = 0.5 / np.pi
     k = kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0)
     vel = k.evaluate(VortexSystem([0.0, 1e-10], [0.0, 0.0], [1.0, -1.0]))
@@ -111,7 +111,7 @@ def test_periodic_close_pairs(oracle, precision, seps, bound, symmetric):
 
 @pytest.mark.parametrize("geometry", ["infinite", "periodic", "disk"])
 def test_hamiltonian_gradient_and_conservation(geometry):
-    """src/audit_tests.cpp:51-85: velocity = (1/G)(dH/dy, -dH/dx) by central differences, and
+    """tests/audit_tests.cpp:51-85: velocity = (1/G)(dH/dy, -dH/dx) by central differences, and
     H conserved over 100 RK4 steps at dt = 1e-4."""
     kw = {"coreRadius": 0.05} if geometry == "infinite" else {}
     if geometry == "periodic":
@@ -141,7 +141,7 @@ def test_hamiltonian_gradient_and_conservation(geometry):
 
 
 def test_invalid_inputs():
-    """src/audit_tests.cpp:99-118 and the reference's validation messages."""
+    """tests/audit_tests.cpp:99-118 and the reference's validation messages."""
     k = kernel_for("infinite")
     with pytest.raises(InvalidArgument, match="vortex data must be finite"):
         k.evaluate(VortexSystem([np.nan], [0.0], [1.0]))
@@ -294,7 +294,7 @@ def test_rk4_advance_matches_single_steps():
 
 
 def test_rk4_pair_rotation():
-    """src/tests.cpp:42-51: co-rotating pair to t=1 at dt=1e-3 (2e-12)."""
+    """tests/tests.cpp:42-51: co-rotating pair to t=1 at dt=1e-3 (2e-12)."""
     k = kernel_for("infinite")
     s = VortexSystem([-1.0, 1.0], [0.0, 0.0], [1.0, 1.0])
     k.uploadDeviceState(s)
@@ -305,7 +305,7 @@ def test_rk4_pair_rotation():
 
 
 def test_dopri_pair_rotation():
-    """src/tests.cpp:52-70: DOPRI5 to t=1 (2e-10)."""
+    """tests/tests.cpp:52-70: DOPRI5 to t=1 (2e-10)."""
     k = kernel_for("infinite")
     s = VortexSystem([-1.0, 1.0], [0.0, 0.0], [1.0, 1.0])
     k.uploadDeviceState(s)
@@ -345,7 +345,7 @@ def test_dopri5_parity(oracle, geometry):
 
 
 def test_dopri_fsal_counts_and_invalidation(oracle):
-    """src/tests.cpp:297-314: the co-rotating pair takes 7 evaluations in its first DOPRI5
+    """tests/tests.cpp:297-314: the co-rotating pair takes 7 evaluations in its first DOPRI5
     step, 13 after the second (the FSAL stage is reused) and 20 after an invalidation (the
     first stage is recomputed), counted by the handle's launch counter.  On a 64-vortex
     population the same bookkeeping holds with rejections (6 more per rejected attempt), and

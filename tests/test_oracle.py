@@ -1,7 +1,7 @@
 """Pins the CPU oracle (oracle/pv_oracle.c) before it is trusted as the checker.
 
 * golden vectors written by the reference's own CPU code (tests/golden/*.npz);
-* the reference's closed-form known-answer tests (src/tests.cpp, src/audit_tests.cpp);
+* the reference's closed-form known-answer tests (tests/tests.cpp, tests/audit_tests.cpp);
 * high-precision brute force (mpmath) of the textbook formulas on tiny inputs: Biot-Savart,
   the circle-theorem image and, for the periodic box, rows of point vortices summed directly;
 * the reference build itself (oracle/_ref), when present, on fresh random inputs;
@@ -102,7 +102,7 @@ def test_oracle_dipole_candidates_match_golden_removal(oracle, geometry):
 
 
 def test_known_answers(oracle):
-    """src/tests.cpp:26-41, 126-153 and src/audit_tests.cpp:27-47 on the oracle."""
+    """tests/tests.cpp:26-41, 126-153 and tests/audit_tests.cpp:27-47 on the oracle."""
     u, v = oracle.evaluate(spec("infinite"), [2.0], [-3.0], [4.0])
     assert u[0] == 0.0 and v[0] == 0.0
     u, v = oracle.evaluate(spec("infinite"), [-1.0, 1.0], [0.0, 0.0], [1.0, 1.0])
@@ -122,7 +122,7 @@ def test_known_answers(oracle):
 
 
 def test_oracle_rk4_pair_rotation(oracle):
-    """src/tests.cpp:42-51."""
+    """tests/tests.cpp:42-51."""
     x, y, g = np.array([-1.0, 1.0]), np.array([0.0, 0.0]), np.array([1.0, 1.0])
     it = oracle.integrator(2)
     for _ in range(1000):
