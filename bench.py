@@ -99,15 +99,20 @@ def per_geometry(pv, n, repeats=3):
             k = pv.HipKernel(pv.SimParams(boundaryCondition=geo, boxLengthX=2 * np.pi,
                                           boxLengthY=2 * np.pi, precision=prec))
             k.uploadDeviceState(pv.VortexSystem(pop.x, pop.y, pop.circulation))
+            k.profileBegin()
             k.evaluateResident(1)
             k.synchronize()
+            one_ms, _ = k.profileEnd()
+            # At least `repeats` evaluations and about 30 ms of them, so that the timing does not
+            # sit on the clock's ramp after the previous kernel.
+            reps = max(repeats, min(50, int(30.0 / max(one_ms, 1e-3))))
             k.profileBegin()
             t0 = time.perf_counter()
-            k.evaluateResident(repeats)
+            k.evaluateResident(reps)
             k.synchronize()
             dt = time.perf_counter() - t0
             ms, launches = k.profileEnd()
-            pairs = repeats * n * (n - 1)
+            pairs = reps * n * (n - 1)
             vel = k.evaluateDeviceState()
             if prec == "fp64":
                 ref_vel = vel
