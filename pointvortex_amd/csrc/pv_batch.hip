@@ -297,7 +297,7 @@ __device__ __forceinline__ void sym_item_of(const BatchLaunch &a, const Instance
 template <class P, int TPT>
 __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const PairItem &it,
                                SymSlice<P, TPT> *sl, int lane, int dir) {
-    using Acc = typename P::Acc;
+    using Acc = typename P::SymAcc;
     constexpr int TT = 64 * TPT;
     const GeoParams &gp = ins->gp;
     const int off = ins->offset, n = ins->n;

@@ -4,7 +4,8 @@
 //   Src   the per-source record staged in LDS (read by every lane of a wave at once,
 //         i.e. an LDS broadcast), with all source-only factors folded in;
 //   Tgt   the per-target record held in registers;
-//   Acc   the per-target accumulators;
+//   Acc   the per-target accumulators; SymAcc those of the symmetric kernels (the mixed
+//         policies fold into a plain FP64 sum there, BlockSum);
 //   pair()          one source's contribution to one target (j != i guaranteed);
 //   pair_checked()  the same, for the few source tiles that contain the target itself;
 //   fold()          called after each LDS tile (mixed: FP32 tile sum -> FP64 TwoSum).
@@ -128,6 +129,25 @@ struct Comp {
     __device__ __forceinline__ double value() const { return hi + lo; }
 };
 
+// Plain FP64 sum.  A symmetric block or ensemble item folds at most 8 FP32 tile sums per
+// target (a 1024-vortex source tile in 128-record chunks; 4 in an ensemble item); 24-bit
+// significands add in FP64 without rounding while their exponents span under 26 binades, and
+// otherwise round at 2^-53 of the largest, far below the FP32 sums' own 2^-24.  The compensated
+// (TwoSum) sum of these per-block results runs in the reduction over the tile rows.
+struct BlockSum {
+    double hi;
+    __device__ __forceinline__ void zero() { hi = 0.0; }
+    __device__ __forceinline__ void add(double x) { hi += x; }
+    __device__ __forceinline__ double value() const { return hi; }
+};
+
+// Per-target accumulators of the mixed policies: FP32 sums over the current tile, folded into
+// S (Comp in the kernels that fold many tiles per target, BlockSum in the symmetric ones).
+template <class S> struct MixedAcc {
+    float u32, v32;
+    S u, v;
+};
+
 // Exact IEEE remainder(d, L) for the common case (the quotient fits comfortably in a double):
 // d - n L with n = d/L rounded to nearest; fma makes the subtraction exact.  A quotient that
 // lands on the other side of a half-integer only moves the result by +-L, to which every
@@ -245,6 +265,7 @@ struct InfF64 {
     struct Acc {
         double u, v;
     };
+    using SymAcc = Acc;
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = true;
     static constexpr bool kNearTiles = false;
@@ -324,10 +345,8 @@ struct InfMixed {
     struct Tgt {
         double x, y;
     };
-    struct Acc {
-        float u32, v32;
-        Comp u, v;
-    };
+    using Acc = MixedAcc<Comp>;
+    using SymAcc = MixedAcc<BlockSum>;
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
@@ -337,19 +356,19 @@ struct InfMixed {
         return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
     }
     __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
-    __device__ static void zero(Acc &a) {
+    template <class A> __device__ static void zero(A &a) {
         a.u32 = a.v32 = 0.f;
         a.u.zero();
         a.v.zero();
     }
-    __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
+    template <class A> __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, A &a) {
         const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, (float)p.eps2));
         const float c = s.g * rcp32(r2);
         a.u32 = __builtin_fmaf(-c, dy, a.u32);
         a.v32 = __builtin_fmaf(c, dx, a.v32);
     }
-    __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, Acc &a,
+    template <class A> __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, A &a,
                                         bool self) {
         const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
         float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, (float)p.eps2));
@@ -358,12 +377,12 @@ struct InfMixed {
         a.u32 = __builtin_fmaf(-c, dy, a.u32);
         a.v32 = __builtin_fmaf(c, dx, a.v32);
     }
-    __device__ static void fold(Acc &a) {
+    template <class A> __device__ static void fold(A &a) {
         a.u.add((double)a.u32);
         a.v.add((double)a.v32);
         a.u32 = a.v32 = 0.f;
     }
-    __device__ static void result(const Acc &a, double &u, double &v) {
+    template <class A> __device__ static void result(const A &a, double &u, double &v) {
         u = a.u.value();
         v = a.v.value();
     }
@@ -379,8 +398,9 @@ struct InfMixed {
     };
     __device__ static float sym_g(const GeoParams &, double g) { return (float)(kInvTwoPi * g); }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
+    template <class A>
     __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, float gi, const Src &s,
-                                    Acc &a, SymR &r) {
+                                    A &a, SymR &r) {
         const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
         const float inv = rcp32(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, (float)p.eps2)));
         const float c = s.g * inv;
@@ -446,6 +466,7 @@ struct DiskF64 {
     struct Acc {
         double u, v;
     };
+    using SymAcc = Acc;
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = true;
     static constexpr bool kNearTiles = true;
@@ -600,10 +621,8 @@ struct DiskMixed {
     struct Tgt {
         double x, y;
     };
-    struct Acc {
-        float u32, v32;
-        Comp u, v;
-    };
+    using Acc = MixedAcc<Comp>;
+    using SymAcc = MixedAcc<BlockSum>;
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
@@ -620,25 +639,25 @@ struct DiskMixed {
         return {x, y, (float)(kInvTwoPi * g), ix, iy, 0.f};
     }
     __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
-    __device__ static void zero(Acc &a) {
+    template <class A> __device__ static void zero(A &a) {
         a.u32 = a.v32 = 0.f;
         a.u.zero();
         a.v.zero();
     }
-    __device__ static void image(const Src &s, float dx, float dy, Acc &a) {
+    template <class A> __device__ static void image(const Src &s, float dx, float dy, A &a) {
         const float ex = dx - s.ix, ey = dy - s.iy;
         const float k = s.g * rcp32(__builtin_fmaf(ex, ex, ey * ey));
         a.u32 = __builtin_fmaf(k, ey, a.u32);
         a.v32 = __builtin_fmaf(-k, ex, a.v32);
     }
-    __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, Acc &a) {
+    template <class A> __device__ static void pair(const GeoParams &, const Tgt &t, const Src &s, A &a) {
         const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
         const float c = s.g * rcp32(__builtin_fmaf(dx, dx, dy * dy));
         a.u32 = __builtin_fmaf(-c, dy, a.u32);
         a.v32 = __builtin_fmaf(c, dx, a.v32);
         image(s, dx, dy, a);
     }
-    __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, Acc &a,
+    template <class A> __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, A &a,
                                         bool self) {
         // The self pair has no direct term; its own image (d = 0) stays.
         const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
@@ -649,12 +668,12 @@ struct DiskMixed {
         a.v32 = __builtin_fmaf(c, dx, a.v32);
         image(s, dx, dy, a);
     }
-    __device__ static void fold(Acc &a) {
+    template <class A> __device__ static void fold(A &a) {
         a.u.add((double)a.u32);
         a.v.add((double)a.v32);
         a.u32 = a.v32 = 0.f;
     }
-    __device__ static void result(const Acc &a, double &u, double &v) {
+    template <class A> __device__ static void result(const A &a, double &u, double &v) {
         u = a.u.value();
         v = a.v.value();
     }
@@ -695,9 +714,9 @@ struct DiskMixed {
         const Src s = src(p, x, y, 0.0);
         return {x, y, s.ix, s.iy, (float)__builtin_fma(x, x, y * y)};
     }
-    template <bool kDirect>
+    template <bool kDirect, class A>
     __device__ static void sym_pair_t(const GeoParams &, const SymTgt &t, float gi, const Src &s,
-                                      Acc &a, SymR &r) {
+                                      A &a, SymR &r) {
         const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
         const float inv = rcp32(__builtin_fmaf(dx, dx, dy * dy));
         const float c = s.g * inv;
@@ -760,6 +779,7 @@ struct PerF64 {
     struct Acc {
         double u, v;
     };
+    using SymAcc = Acc;
     static constexpr int kFoldEvery = 0;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
@@ -864,10 +884,8 @@ struct PerMixed {
     struct Tgt {
         double x, y;
     };
-    struct Acc {
-        float u32, v32;
-        Comp u, v;
-    };
+    using Acc = MixedAcc<Comp>;
+    using SymAcc = MixedAcc<BlockSum>;
     static constexpr int kFoldEvery = 1;
     static constexpr bool kBatched = false;
     static constexpr bool kNearTiles = false;
@@ -877,7 +895,7 @@ struct PerMixed {
         return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
     }
     __device__ static Tgt tgt(const GeoParams &, double x, double y) { return {x, y}; }
-    __device__ static void zero(Acc &a) {
+    template <class A> __device__ static void zero(A &a) {
         a.u32 = a.v32 = 0.f;
         a.u.zero();
         a.v.zero();
@@ -924,13 +942,13 @@ struct PerMixed {
         fu = __builtin_fmaf(sin_y * nu, rcp32(du * qu), au);
         fv = __builtin_fmaf(sin_x * nv, rcp32(dv * qv), av);
     }
-    __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, Acc &a) {
+    template <class A> __device__ static void pair(const GeoParams &p, const Tgt &t, const Src &s, A &a) {
         float fu, fv;
         terms(p, t, s, fu, fv, false);
         a.u32 = __builtin_fmaf(-s.gs, fu, a.u32);
         a.v32 = __builtin_fmaf(s.gs, fv, a.v32);
     }
-    __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, Acc &a,
+    template <class A> __device__ static void pair_checked(const GeoParams &p, const Tgt &t, const Src &s, A &a,
                                         bool self) {
         float fu, fv;
         terms(p, t, s, fu, fv, self);
@@ -948,8 +966,9 @@ struct PerMixed {
     };
     __device__ static float sym_g(const GeoParams &p, double g) { return (float)(p.scale * g); }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
+    template <class A>
     __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, float gi, const Src &s,
-                                    Acc &a, SymR &r) {
+                                    A &a, SymR &r) {
         float fu, fv;
         terms(p, t, s, fu, fv, false);
         a.u32 = __builtin_fmaf(-s.gs, fu, a.u32);
@@ -957,12 +976,12 @@ struct PerMixed {
         r.u = __builtin_fmaf(gi, fu, r.u);
         r.v = __builtin_fmaf(-gi, fv, r.v);
     }
-    __device__ static void fold(Acc &a) {
+    template <class A> __device__ static void fold(A &a) {
         a.u.add((double)a.u32);
         a.v.add((double)a.v32);
         a.u32 = a.v32 = 0.f;
     }
-    __device__ static void result(const Acc &a, double &u, double &v) {
+    template <class A> __device__ static void result(const A &a, double &u, double &v) {
         u = a.u.value();
         v = a.v.value();
     }

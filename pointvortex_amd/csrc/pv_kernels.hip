@@ -256,7 +256,7 @@ __global__ __launch_bounds__(sym_block(TPT))
 __attribute__((amdgpu_waves_per_eu(TPT >= 8 ? 3 : 1))) void sym_kernel(SymLaunch a, GeoParams gp) {
     if (a.st->fail_epoch < a.epoch)
         return;
-    using Acc = typename P::Acc;
+    using Acc = typename P::SymAcc;
     __shared__ typename P::Src sh[(kRingWindow<P> ? 2 : 1) * TILE];
     constexpr int B = sym_block(TPT);
     __shared__ double red[B / 64][TILE][2];

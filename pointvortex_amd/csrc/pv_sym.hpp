@@ -58,7 +58,7 @@ __device__ __forceinline__ typename P::Src ring_src(const GeoParams &gp, double 
 template <class P, int TPT, bool kDirect = false>
 __device__ __forceinline__ void sym_step(const GeoParams &gp, const typename P::SymTgt *tg,
                                          const typename P::SymScalar *gi,
-                                         const typename P::Src &s, typename P::Acc *acc,
+                                         const typename P::Src &s, typename P::SymAcc *acc,
                                          typename P::SymR &r) {
     if constexpr (P::kSymDenoms > 0) {
         constexpr int ND = P::kSymDenoms;
