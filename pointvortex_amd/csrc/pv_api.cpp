@@ -401,6 +401,7 @@ void evaluate_symmetric(pv_handle *h, const double *x, const double *y, const do
     a.part = h->part;
     a.st = h->st;
     a.epoch = ++h->epoch;
+    a.self_image = h->cfg.geometry == PV_DISK && h->cfg.precision == PV_MIXED;
     a.op = op;
     h->records.push_back({x, y, (int)n, 0, (int)n});
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

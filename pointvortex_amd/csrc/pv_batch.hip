@@ -141,6 +141,8 @@ __device__ void batch_item(const BatchLaunch &a, const Instance *ins, const Work
     double u, v;
     P::result(acc, u, v);
     if (valid) {
+        if constexpr (kSelfImage<P>)
+            disk_self_image(gp, xt, yt, a.g[gi], u, v);
         const double dt = ADAPT ? stp->dt : a.op.dt;
         const double atol = ADAPT ? stp->atol : a.op.atol;
         const double rtol = ADAPT ? stp->rtol : a.op.rtol;
@@ -535,11 +537,13 @@ __global__ __launch_bounds__(kBatchBlock) void batch_sym_reduce_kernel(BatchLaun
         su.add(__builtin_nontemporal_load(p + (size_t)(2 * s) * nn));
         sv.add(__builtin_nontemporal_load(p + (size_t)(2 * s + 1) * nn));
     }
+    double u = su.value(), w = sv.value();
+    if (ins.gp.geometry == 2 && ins.precision == 1)
+        disk_self_image(ins.gp, xt, yt, a.g[v], u, w);
     const double dt = ADAPT ? stp->dt : a.op.dt;
     const double atol = ADAPT ? stp->atol : a.op.atol;
     const double rtol = ADAPT ? stp->rtol : a.op.rtol;
-    const double err =
-        stage_epilogue(a.op, dt, atol, rtol, fail, a.x, a.y, v, su.value(), sv.value());
+    const double err = stage_epilogue(a.op, dt, atol, rtol, fail, a.x, a.y, v, u, w);
     if (ADAPT && a.op.kind == kOpDopriError)
         atomicMax(&stp->err_bits, (unsigned long long)__double_as_longlong(err));
 }

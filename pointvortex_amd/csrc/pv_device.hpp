@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pv {
 
 constexpr double kPi = 3.14159265358979323846264338327950288;
@@ -644,9 +646,12 @@ struct DiskMixed {
         a.u.zero();
         a.v.zero();
     }
-    template <class A> __device__ static void image(const Src &s, float dx, float dy, A &a) {
+    // j's image on i; `self` (j == i) leaves the term out: disk_self_image adds it in FP64.
+    template <class A>
+    __device__ static void image(const Src &s, float dx, float dy, A &a, bool self = false) {
         const float ex = dx - s.ix, ey = dy - s.iy;
-        const float k = s.g * rcp32(__builtin_fmaf(ex, ex, ey * ey));
+        float k = s.g * rcp32(__builtin_fmaf(ex, ex, ey * ey));
+        k = self ? 0.f : k;
         a.u32 = __builtin_fmaf(k, ey, a.u32);
         a.v32 = __builtin_fmaf(-k, ex, a.v32);
     }
@@ -659,14 +664,15 @@ struct DiskMixed {
     }
     template <class A> __device__ static void pair_checked(const GeoParams &, const Tgt &t, const Src &s, A &a,
                                         bool self) {
-        // The self pair has no direct term; its own image (d = 0) stays.
+        // The self pair has no direct term, and its own image (d = 0) is added in FP64 where
+        // the target's velocity is finalized (disk_self_image), outside the FP32 tile sums.
         const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
         float r2 = __builtin_fmaf(dx, dx, dy * dy);
         r2 = self ? 1.f : r2;
         const float c = self ? 0.f : s.g * rcp32(r2);
         a.u32 = __builtin_fmaf(-c, dy, a.u32);
         a.v32 = __builtin_fmaf(c, dx, a.v32);
-        image(s, dx, dy, a);
+        image(s, dx, dy, a, self);
     }
     template <class A> __device__ static void fold(A &a) {
         a.u.add((double)a.u32);
@@ -748,6 +754,27 @@ struct DiskMixed {
         }
     }
 };
+
+// The mixed disk's self-image term: vortex i's own image (src/compute.cpp:320-329 with j = i),
+// the velocity of -G_i at x_i* on x_i, (G_i / 2 pi) (-delta_y, delta_x) / |delta|^2 with
+// delta = x_i* - x_i (image_offset).  Near the wall it is a vortex's largest term (|delta| is
+// about twice the gap), and inside an FP32 tile sum every later addition would round at its
+// scale (a vortex 5e-4 from the wall: 1.2e-6 relative error in its velocity).  DiskMixed's
+// pair_checked therefore leaves it out, and every kernel that finalizes a mixed disk target's
+// velocity adds it here, in FP64, once.  A vortex within 1e-30 R of the centre has an image
+// term below 1e-30 of its own scale and gets none (src() gives its image a far offset).
+__device__ __forceinline__ void disk_self_image(const GeoParams &p, double x, double y, double g,
+                                                double &u, double &v) {
+    const double r2 = __builtin_fma(x, x, y * y);
+    if (!(r2 >= 1e-60 * p.R2))
+        return;
+    double ox, oy;
+    image_offset(p, x, y, r2, ox, oy);
+    const double k = kInvTwoPi * g / __builtin_fma(ox, ox, oy * oy);
+    u = __builtin_fma(-k, oy, u);
+    v = __builtin_fma(k, ox, v);
+}
+template <class P> constexpr bool kSelfImage = std::is_same_v<P, DiskMixed>;
 
 // ---------------------------------------------------------------- periodic box
 

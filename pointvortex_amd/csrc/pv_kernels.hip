@@ -189,9 +189,13 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
 
     double err = 0.0;
 #pragma unroll
-    for (int t = 0; t < TPT; ++t)
-        if (ti[t] >= 0)
+    for (int t = 0; t < TPT; ++t) {
+        if (ti[t] >= 0) {
+            if constexpr (kSelfImage<P>)
+                disk_self_image(gp, tg[t].x, tg[t].y, a.g[ti[t]], u[t], v[t]);
             err = fmax(err, epilogue(a, gp, ti[t], u[t], v[t]));
+        }
+    }
     if (a.op.kind == kOpDopriError) {
         const double m = block_max(err);
         if (tid == 0)
@@ -515,9 +519,11 @@ __global__ __launch_bounds__(kBlock) void sym_reduce_kernel(SymLaunch a, GeoPara
             su.add(__builtin_nontemporal_load(&a.part[(size_t)(2 * s) * n + i]));
             sv.add(__builtin_nontemporal_load(&a.part[(size_t)(2 * s + 1) * n + i]));
         }
+        double u = su.value(), v = sv.value();
+        if (a.self_image)
+            disk_self_image(gp, x, y, a.g[i], u, v);
         err = stage_epilogue(a.op, a.op.dt, a.op.atol, a.op.rtol,
-                             FailSink{a.st, a.epoch, nullptr}, a.x, a.y, i, su.value(),
-                             sv.value());
+                             FailSink{a.st, a.epoch, nullptr}, a.x, a.y, i, u, v);
     }
     if (a.op.kind == kOpDopriError) {
         const double m = block_max(err);

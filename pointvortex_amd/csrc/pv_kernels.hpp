@@ -67,6 +67,7 @@ struct SymLaunch {
     double *part;              // [nt][2][n]
     DevStatus *st;
     unsigned epoch;
+    int self_image;            // 1 for the mixed disk: the reduction adds disk_self_image
     StageOp op;
 };
 

@@ -250,8 +250,11 @@ __global__ __launch_bounds__(kPersistBlock) void rk4_persist_kernel(RkPersistLau
             red[tid][1] = v;
             __syncthreads();
             persist_reduce(red, runs, tb, G, u, v);
-            if (valid && q == 0)
+            if (valid && q == 0) {
+                if constexpr (kSelfImage<P>)
+                    disk_self_image(gp, xi, yi, a.g[i], u, v);
                 persist_epilogue(op, c, FailSink{a.st, epoch, nullptr}, i, u, v, own);
+            }
             if (s == a.steps - 1 && c == 3)
                 return;
             arrivals += gridDim.x;

@@ -543,6 +543,46 @@ def test_mixed_disk_run_against_the_oracle_at_first_and_last_step(oracle):
             assert normwise(vel.x[b:e], vel.y[b:e], ur[b:e], vr[b:e]) <= 1e-6, (b, e)
 
 
+def test_mixed_disk_self_image_stays_out_of_the_fp32_sums(oracle):
+    """128 vortices uniform in the unit disk, one of them 5.5e-4 from the wall: its velocity
+    (about 146, against an rms of 15) is nearly all its own image.  Inside an FP32 tile sum
+    that term made every later FP32 addition round at its scale, 1.0e-6 normwise against FP64
+    on this population.  The self-image term is added in FP64 where each target's velocity is
+    finalized, so every path that evaluates the mixed disk stays well inside the bar: the
+    split and symmetric single-system paths, both ensemble decompositions, and one RK4 step
+    of the persistent kernel (displacement against the oracle's step)."""
+    from pointvortex_amd.ensemble import Ensemble, InstanceConfig
+
+    pop = wl.disk(128, 17 * 1_000_003 + 1083, wall_fraction=0.0, gaussian=False)
+    r = np.hypot(pop.x, pop.y)
+    assert 5e-4 < 1.0 - r.max() < 6e-4
+    ur, vr = oracle.evaluate(spec("disk"), pop.x, pop.y, pop.circulation)
+    bar = 5e-7
+    for shape in ((1, 1, 128, 0), (1, 0, 128, 1), (2, 0, 128, 1)):
+        k = kernel_for("disk", "mixed")
+        k.set_launch(*shape)
+        vel = k.evaluate(VortexSystem(pop.x, pop.y, pop.circulation))
+        assert normwise(vel.x, vel.y, ur, vr) <= bar, shape
+    for symmetric in (0, 1):
+        e = Ensemble(2 * pop.n, 2, "mixed")
+        e.set_symmetric(symmetric)
+        cfg = InstanceConfig("disk", disk_radius=1.0)
+        e.load([cfg, cfg], [pop.x, pop.x], [pop.y, pop.y], [pop.circulation] * 2)
+        us, vs = e.evaluate()
+        e.close()
+        for q in range(2):
+            assert normwise(us[q], vs[q], ur, vr) <= bar, (symmetric, q)
+    dt = 1e-6
+    x, y = pop.x.copy(), pop.y.copy()
+    oracle.integrator(pop.n).rk4_step(spec("disk"), x, y, pop.circulation, dt)
+    k = kernel_for("disk", "mixed")
+    s = VortexSystem(pop.x, pop.y, pop.circulation)
+    k.uploadDeviceState(s)
+    k.deviceRk4Step(dt)
+    k.downloadDeviceState(s)
+    assert normwise(s.x - pop.x, s.y - pop.y, x - pop.x, y - pop.y) <= bar
+
+
 def _planted(geometry, seed):
     """300 vortices with four opposite-sign pairs moved to within 3e-4 of each other (and, in
     the disk, three vortices within 2e-4 of the wall)."""
