@@ -418,7 +418,7 @@ def serving_bench(pv, count=128, seed=5, kw=SERVING, budget=4 << 30, max_wall=12
 
 def serving_wide_bench(pv):
     return serving_bench(pv, count=64, seed=7, kw=SERVING_WIDE, budget=256 << 30,
-                         max_wall=90.0, name="serving wide", every=True,
+                         max_wall=150.0, name="serving wide", every=True,
                          top=((131072, "infinite"), (65536, "disk")))
 
 

@@ -91,15 +91,21 @@ class ServerStats:
     attempts: int = 0
 
     def summary(self):
-        lat = np.array([r.latency for r in self.results if r.state == "finished"])
-        done = len(lat)
+        """Latency percentiles over the finished runs and the unfinished ones, whose latency
+        counts as the time they had waited when the loop stopped (a lower bound): with any
+        unfinished run the percentiles are lower bounds too (`latency_lower_bound`)."""
+        done = sum(r.state == "finished" for r in self.results)
+        unfinished = sum(r.state == "unfinished" for r in self.results)
+        lat = np.array([r.latency for r in self.results if r.state in ("finished", "unfinished")])
         return {
             "completed": done,
             "failed": sum(r.state == "failed" for r in self.results),
+            "unfinished": unfinished,
             "wall_s": self.wall,
             "solves_per_s": done / self.wall if self.wall > 0 else 0.0,
-            "p50_s": float(np.percentile(lat, 50)) if done else None,
-            "p99_s": float(np.percentile(lat, 99)) if done else None,
+            "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+            "latency_lower_bound": unfinished > 0,
             "attempts": self.attempts,
         }
 
@@ -221,6 +227,17 @@ class Server:
                 if on_result is not None:
                     on_result(stats.results[-1], stats)
         stats.wall = clock() - t0
+        # Runs still going, or not yet admitted, when max_wall stopped the loop: recorded as
+        # unfinished, their latency measured up to now (a lower bound of the real one).
+        counts = {s.slot: (s.accepted, s.rejected) for s in eng.poll() if s.slot in active}
+        for slot, (req, admitted, prec) in active.items():
+            acc, rej = counts.get(slot, (0, 0))
+            stats.results.append(Result(req.rid, "unfinished", req.arrival, admitted, stats.wall,
+                                        acc, rej, "", prec))
+        for req in pending:
+            stats.results.append(Result(req.rid, "unfinished", req.arrival, float("nan"),
+                                        max(stats.wall, req.arrival), 0, 0, "",
+                                        self.precision_for(req)))
         return stats
 
 
