@@ -41,7 +41,9 @@ typedef enum pv_geometry { PV_INFINITE = 0, PV_PERIODIC = 1, PV_DISK = 2 } pv_ge
 /* Arithmetic of the pair sums.  PV_FP64 is the drop-in (matches the reference's FP64 CPU
  * path to rounding).  PV_MIXED forms pair differences in FP64, evaluates the pair terms in
  * FP32 and folds per-tile FP32 partial sums into FP64 with compensated (TwoSum)
- * accumulation; it has no counterpart in the reference. */
+ * accumulation (in the symmetric path: a plain FP64 sum of a block's few tile sums, then
+ * TwoSum over the blocks); the disk's self-image term is added in FP64.  It has no
+ * counterpart in the reference. */
 typedef enum pv_precision { PV_FP64 = 0, PV_MIXED = 1 } pv_precision;
 
 /* Kernel parameters: the geometry fields of SimParams (src/params.h:44-57) that
