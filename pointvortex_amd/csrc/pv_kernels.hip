@@ -207,13 +207,17 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
 
 // The infinite-plane policies, whose pair is short enough that the ring's data movement is a
 // sizeable share of a step: their symmetric kernels also have 8 targets per thread.  The FP64
-// one's ring reads its sources from a doubled window (win), and its diagonal blocks share one
-// reciprocal over two sources x TPT targets, as the split kernel does.  (In the mixed kernel
-// the window costs more than it saves: the compiler then loads each source in the step that
-// uses it instead of one step ahead.)
+// infinite and disk rings read their sources from a doubled window (win), one index add per
+// step instead of three; the infinite FP64 diagonal blocks also share one reciprocal over two
+// sources x TPT targets, as the split kernel does.  (In the mixed kernels the window costs
+// more than it saves: the compiler then loads each source in the step that uses it instead of
+// one step ahead.)
 template <class P>
 constexpr bool kInfPlane = std::is_same_v<P, InfF64> || std::is_same_v<P, InfMixed>;
-template <class P> constexpr bool kRingWindow = std::is_same_v<P, InfF64>;
+template <class P>
+constexpr bool kRingWindow = std::is_same_v<P, InfF64> || std::is_same_v<P, DiskF64>;
+// The diagonal blocks' shared reciprocal over two sources (infinite FP64 only).
+template <class P> constexpr bool kDiagShared = std::is_same_v<P, InfF64>;
 
 // Slot of source k of a staged chunk: with the window, each group of 64 sources is staged twice
 // in a row (group k / 64 at 128 (k / 64)), so that the ring's index needs no wrap.
@@ -322,7 +326,7 @@ __attribute__((amdgpu_waves_per_eu(TPT >= 8 ? 3 : 1))) void sym_kernel(SymLaunch
                 sh[k] = P::src(gp, a.x[base + k], a.y[base + k], a.g[base + k]);
             __syncthreads();
             int k0 = 0;
-            if constexpr (kRingWindow<P>) {
+            if constexpr (kDiagShared<P>) {
                 // Q sources x TPT targets share one rcp64 (Q = 2, or 1 at 8 targets per thread).
                 // The self pair has zero difference and so no term; its denominator (and a
                 // padding lane's) is set to 1 so that the product stays usable.
