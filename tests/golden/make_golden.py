@@ -6,6 +6,10 @@ Run in the build container, where the reference tree is present:
 Each fixture holds the seeded inputs and the reference's outputs: velocities (evaluate),
 invariants (computeInvariants), 10 fixed RK4 steps, a short adaptive DOPRI5 run with its
 accepted-step records, and the population after one DipoleManager::process call.
+reference_periodic_state.npz holds the same outputs, dipoles aside (its clusters have no
+opposite-sign pair closer than 0.08), for a 200-vortex periodic state the reference keeps
+among its plotting examples (`python tests/golden/make_golden.py state` writes only that
+one).
 """
 import sys
 from pathlib import Path
@@ -72,5 +76,32 @@ def main():
         print(geometry, sorted(data)[:4], "...", len(data), "arrays")
 
 
+# A state the reference itself holds: 200 vortices of +-0.005 in the 2 pi periodic box, from
+# its plotting examples (scripts/plotting/vortex_state/vortex_xy.006500; x y circulation per
+# line).  Only its numbers are kept, with the reference's outputs for them.
+STATE = Path("/root/reference/scripts/plotting/vortex_state/vortex_xy.006500")
+TWO_PI = 2.0 * np.pi
+
+
+def periodic_state():
+    ref = Reference()
+    a = np.loadtxt(STATE)
+    x, y, g = (np.ascontiguousarray(a[:, c]) for c in range(3))
+    k = spec("periodic", box_length=TWO_PI, image_layers=8)
+    data = {"x": x, "y": y, "g": g}
+    data["u"], data["v"] = ref.evaluate(k, x, y, g)
+    data["invariants"] = ref.invariants(k, x, y, g)
+    data["rk4_dt"] = np.array([1e-2])
+    data["rk4_x"], data["rk4_y"] = ref.rk4(k, x, y, g, 1e-2, 10)
+    xd, yd, rec = ref.dopri5_run(k, x, y, g, 1e-2, 0.5, 1e-12, 1e-9, 1e-12, 0.1)
+    data["dopri_x"], data["dopri_y"], data["dopri_record"] = xd, yd, rec
+    np.savez_compressed(OUT / "reference_periodic_state.npz", **data)
+    print("periodic state", len(x), "vortices,", len(rec), "DOPRI5 steps")
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "state":
+        periodic_state()
+    else:
+        main()
+        periodic_state()

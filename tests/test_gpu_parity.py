@@ -277,6 +277,46 @@ def test_rk4_steps_parity(oracle, geometry):
     assert np.max(np.abs(out.y - y)) <= 1e-12 * scale
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_reference_periodic_state(precision):
+    """A state the reference keeps among its plotting examples: 200 vortices of +-0.005 in the
+    2 pi box, 8 image layers, against the reference's own outputs (tests/golden): velocities
+    (1e-12 FP64, 1e-6 mixed); in FP64 also 10 RK4 steps at dt = 1e-2 (1e-12 of the coordinate
+    scale) and a DOPRI5 run to t = 0.5 with its accepted-step records (the same steps,
+    accepted dt to 1e-7 relative, and the positions to 1e-10 of the coordinate scale)."""
+    from pathlib import Path
+
+    d = np.load(Path(__file__).resolve().parent / "golden" / "reference_periodic_state.npz")
+    x0, y0, g = d["x"], d["y"], d["g"]
+    kw = dict(boxLengthX=TWO_PI, boxLengthY=TWO_PI, periodicImageLayers=8)
+    k = kernel_for("periodic", precision, **kw)
+    vel = k.evaluate(VortexSystem(x0, y0, g))
+    assert normwise(vel.x, vel.y, d["u"], d["v"]) <= (1e-12 if precision == "fp64" else 1e-6)
+    if precision == "mixed":
+        return
+    scale = np.max(np.abs(np.concatenate([x0, y0])))
+    k.uploadDeviceState(VortexSystem(x0, y0, g))
+    for _ in range(10):
+        k.deviceRk4Step(float(d["rk4_dt"][0]))
+    out = k.downloadDeviceSystem()
+    assert np.max(np.abs(out.x - d["rk4_x"])) <= 1e-12 * scale
+    assert np.max(np.abs(out.y - d["rk4_y"])) <= 1e-12 * scale
+    k = kernel_for("periodic", **kw)
+    k.uploadDeviceState(VortexSystem(x0, y0, g))
+    rec = d["dopri_record"]
+    time, dt = 0.0, 1e-2
+    for row in rec:
+        r = k.deviceDopri5Step(min(dt, 0.5 - time), 1e-12, 1e-9, 1e-12, 0.1)
+        assert r.rejectedSteps == int(row[3])
+        assert abs(r.acceptedTimeStep - row[0]) <= 1e-7 * row[0]
+        time += r.acceptedTimeStep
+        dt = r.suggestedTimeStep
+    assert abs(time - 0.5) <= 1e-12
+    out = k.downloadDeviceSystem()
+    assert np.max(np.abs(out.x - d["dopri_x"])) <= 1e-10 * scale
+    assert np.max(np.abs(out.y - d["dopri_y"])) <= 1e-10 * scale
+
+
 def test_rk4_advance_matches_single_steps():
     pop = wl.infinite(1024, 0)
     a = kernel_for("infinite")

@@ -74,6 +74,33 @@ def test_oracle_integrators_match_golden(oracle, geometry, n):
     assert np.max(np.abs(x - d[f"{t}_dopri_x"])) <= 1e-15
 
 
+def test_oracle_matches_the_reference_on_its_periodic_state(oracle):
+    """A state the reference keeps among its plotting examples (200 vortices of +-0.005 in the
+    2 pi box, 8 image layers): velocities, invariants, 10 RK4 steps at dt = 1e-2 and a DOPRI5
+    run to t = 0.5 with its accepted-step records, as the reference's own code computes them."""
+    d = np.load(GOLDEN / "reference_periodic_state.npz")
+    k = spec("periodic", box_length=2.0 * np.pi, image_layers=8)
+    x0, y0, g = d["x"], d["y"], d["g"]
+    u, v = oracle.evaluate(k, x0, y0, g)
+    assert normwise(u, v, d["u"], d["v"]) <= 1e-15
+    assert np.allclose(oracle.invariants(k, x0, y0, g), d["invariants"], rtol=1e-14, atol=1e-15)
+    x, y = x0.copy(), y0.copy()
+    it = oracle.integrator(len(x))
+    for _ in range(10):
+        it.rk4_step(k, x, y, g, float(d["rk4_dt"][0]))
+    assert np.max(np.abs(x - d["rk4_x"])) <= 1e-15 and np.max(np.abs(y - d["rk4_y"])) <= 1e-15
+    x, y = x0.copy(), y0.copy()
+    it = oracle.integrator(len(x))
+    time, dt, rows = 0.0, 1e-2, []
+    while time < 0.5:
+        r = it.dopri5_step(k, x, y, g, min(dt, 0.5 - time), 1e-12, 1e-9, 1e-12, 0.1)
+        rows.append(r)
+        time += r[0]
+        dt = r[1]
+    assert np.allclose(np.array(rows), d["dopri_record"], rtol=1e-12, atol=0)
+    assert np.max(np.abs(x - d["dopri_x"])) <= 1e-14
+
+
 @pytest.mark.parametrize("geometry", GEOMETRIES)
 def test_oracle_dipole_candidates_match_golden_removal(oracle, geometry):
     """The pairs the reference removed are exactly the oracle's closest-first disjoint
