@@ -76,8 +76,10 @@ def test_close_encounters():
     """tests/audit_tests.cpp:27-47."""
     inv2pi = 0.5 / np.pi
     k = kernel_for("periodic", boxLengthX=2.0, boxLengthY=2.0)
-    vel = k.evaluate(VortexSystem([0.0, 1e-10], [0.0, 0.0], [1.0, -1.0]))
+    pair = VortexSystem([0.0, 1e-10], [0.0, 0.0], [1.0, -1.0])
+    vel = k.evaluate(pair)
     assert abs(vel.y[0] / (inv2pi / 1e-10) - 1.0) <= 1e-12
+    assert np.isfinite(k.hamiltonian(pair))           # audit_tests.cpp:35-36
     with pytest.raises(BackendRuntimeError, match="coincident periodic vortices"):
         k.evaluate(VortexSystem([0.0, 0.0], [0.0, 0.0], [1.0, -1.0]))
     d = kernel_for("disk", diskRadius=1.0)
