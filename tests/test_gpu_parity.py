@@ -671,6 +671,44 @@ def test_dipole_process_matches_reference(reference, geometry, mode):
     assert np.array_equal(out.circulation, rg)
 
 
+def test_dipole_known_answers():
+    """The reference's dipole unit tests (tests/tests.cpp:169-296) on the device path, with
+    the default generator seed 1234567 (src/params.h) unless the case sets one."""
+    def run(geometry, x, y, g, distance, mode, seed=1234567, box=2.0):
+        k = kernel_for(geometry, boxLengthX=box, boxLengthY=box, diskRadius=1.0)
+        k.uploadDeviceState(VortexSystem(x, y, g))
+        k.setDipoleRandom(seed)
+        return k.processDipoles(distance, mode), k.downloadDeviceSystem()
+
+    # Closest-first removal: the (0, 0.01) pair goes, the vortex at 0.02 stays.
+    ev, s = run("infinite", [0.0, 0.01, 0.02], [0.0, 0.0, 0.0], [1.0, -1.0, -1.0], 0.05, "none")
+    assert ev == 1 and s.size() == 1 and s.x[0] == 0.02
+    # Periodic minimum image across the box edge, paired reinjection inside the box.
+    ev, s = run("periodic", [0.99, -0.99, -0.5, 0.5], [0.0, 0.0, 0.5, -0.5],
+                [1.0, -1.0, 1.0, -1.0], 0.05, "paired", seed=42)
+    assert ev == 1 and s.size() == 4
+    assert np.all(np.abs(s.x) <= 1.0) and np.all(np.abs(s.y) <= 1.0)
+    dx = np.remainder(s.x[2] - s.x[3] + 1.0, 2.0) - 1.0
+    dy = np.remainder(s.y[2] - s.y[3] + 1.0, 2.0) - 1.0
+    assert abs(dx * dx + dy * dy - 1.0) <= 2e-15
+    # Disk, independent reinjection keeps the circulations and stays inside the disk.
+    ev, s = run("disk", [0.0, 0.01], [0.0, 0.0], [2.0, -0.5], 0.1, "independent")
+    assert s.size() == 2 and list(s.circulation) == [2.0, -0.5]
+    assert np.all(s.x * s.x + s.y * s.y < 1.0)
+    # Disk, paired reinjection: the pair comes back at squared spacing pi / 2.
+    ev, s = run("disk", [0.0, 0.01], [0.0, 0.0], [1.0, -1.0], 0.1, "paired")
+    d2 = (s.x[0] - s.x[1]) ** 2 + (s.y[0] - s.y[1]) ** 2
+    assert abs(d2 - np.pi / 2.0) <= 3e-15
+    assert np.all(s.x * s.x + s.y * s.y < 1.0)
+    # A vortex 0.02 from the wall forms a dipole with its own image and is removed; with
+    # paired reinjection the population keeps its size and circulation, inside the disk.
+    ev, s = run("disk", [0.98, 0.0], [0.0, 0.0], [1.0, -1.0], 0.05, "none")
+    assert ev == 1 and s.size() == 1 and s.circulation[0] == -1.0
+    ev, s = run("disk", [0.98], [0.0], [2.0], 0.05, "paired")
+    assert ev == 1 and s.size() == 1 and s.circulation[0] == 2.0
+    assert s.x[0] ** 2 + s.y[0] ** 2 < 1.0
+
+
 def test_dipole_generator_state_round_trip(reference):
     """The generator's text state (as checkpoints store it) continues the sequence: a second
     handle restored from it reinjects exactly as the first handle does next."""
