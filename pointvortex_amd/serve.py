@@ -93,14 +93,17 @@ class ServerStats:
     def summary(self):
         """Latency percentiles over the finished runs and the unfinished ones, whose latency
         counts as the time they had waited when the loop stopped (a lower bound): with any
-        unfinished run the percentiles are lower bounds too (`latency_lower_bound`)."""
+        unfinished run the percentiles are lower bounds too (`latency_lower_bound`).  Requests
+        whose arrival time the loop never reached (`not_arrived`) have no latency."""
         done = sum(r.state == "finished" for r in self.results)
         unfinished = sum(r.state == "unfinished" for r in self.results)
+        not_arrived = sum(r.state == "not_arrived" for r in self.results)
         lat = np.array([r.latency for r in self.results if r.state in ("finished", "unfinished")])
         return {
             "completed": done,
             "failed": sum(r.state == "failed" for r in self.results),
             "unfinished": unfinished,
+            "not_arrived": not_arrived,
             "wall_s": self.wall,
             "solves_per_s": done / self.wall if self.wall > 0 else 0.0,
             "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
@@ -234,10 +237,11 @@ class Server:
             acc, rej = counts.get(slot, (0, 0))
             stats.results.append(Result(req.rid, "unfinished", req.arrival, admitted, stats.wall,
                                         acc, rej, "", prec))
+        # Requests whose arrival time the loop never reached are not latencies at all.
         for req in pending:
-            stats.results.append(Result(req.rid, "unfinished", req.arrival, float("nan"),
-                                        max(stats.wall, req.arrival), 0, 0, "",
-                                        self.precision_for(req)))
+            state = "unfinished" if req.arrival <= stats.wall else "not_arrived"
+            stats.results.append(Result(req.rid, state, req.arrival, float("nan"), stats.wall,
+                                        0, 0, "", self.precision_for(req)))
         return stats
 
 

@@ -25,9 +25,11 @@ def test_unfinished_runs_enter_the_percentiles_as_lower_bounds():
     res = [_result(k, "finished", 0.0, 1.0) for k in range(9)]
     res.append(_result(9, "unfinished", 2.0, 90.0))     # stopped by the cap after 88 s
     res.append(_result(10, "failed", 0.0, 0.3))          # failures stay out of the latencies
+    res.append(_result(11, "not_arrived", 95.0, 90.0))   # never arrived: no latency
     st = ServerStats(results=res, wall=90.0, attempts=100)
     s = st.summary()
     assert s["completed"] == 9 and s["unfinished"] == 1 and s["failed"] == 1
+    assert s["not_arrived"] == 1
     assert s["latency_lower_bound"]
     assert np.isclose(s["p99_s"], np.percentile([1.0] * 9 + [88.0], 99))
     assert s["p99_s"] > 80.0
