@@ -354,6 +354,7 @@ struct InfMixed {
     static constexpr bool kNearTiles = false;
     static constexpr bool kSymNear = false;
     static constexpr int kRingUnroll = 1;
+    static constexpr bool kSymTwoPhase = false;
     __device__ static Src src(const GeoParams &, double x, double y, double g) {
         return {x, y, (float)(kInvTwoPi * g), 0.f, 0.f, 0.f};
     }
@@ -400,10 +401,17 @@ struct InfMixed {
     };
     __device__ static float sym_g(const GeoParams &, double g) { return (float)(kInvTwoPi * g); }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
-    template <class A>
-    __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, float gi, const Src &s,
-                                    A &a, SymR &r) {
-        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+    // The pair in two parts, the FP64 difference (sym_diff) and the FP32 rest (sym_rest), so
+    // that a policy's ring step can issue every target's difference before any FP32 work
+    // (sym_step, kSymTwoPhase).  kDirect only selects a form in DiskMixed.
+    __device__ static void sym_diff(const GeoParams &, const SymTgt &t, const Src &s, float &dx,
+                                    float &dy) {
+        dx = (float)(t.x - s.x);
+        dy = (float)(t.y - s.y);
+    }
+    template <bool kDirect, class A>
+    __device__ static void sym_rest(const GeoParams &p, const SymTgt &, float gi, const Src &s,
+                                    float dx, float dy, A &a, SymR &r) {
         const float inv = rcp32(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, (float)p.eps2)));
         const float c = s.g * inv;
         a.u32 = __builtin_fmaf(-c, dy, a.u32);
@@ -630,6 +638,7 @@ struct DiskMixed {
     static constexpr bool kNearTiles = false;
     static constexpr bool kSymNear = true;
     static constexpr int kRingUnroll = 2;
+    static constexpr bool kSymTwoPhase = true;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         const double r2 = __builtin_fma(x, x, y * y);
         float ix = 1e30f, iy = 0.f;
@@ -720,10 +729,14 @@ struct DiskMixed {
         const Src s = src(p, x, y, 0.0);
         return {x, y, s.ix, s.iy, (float)__builtin_fma(x, x, y * y)};
     }
+    __device__ static void sym_diff(const GeoParams &, const SymTgt &t, const Src &s, float &dx,
+                                    float &dy) {
+        dx = (float)(t.x - s.x);
+        dy = (float)(t.y - s.y);
+    }
     template <bool kDirect, class A>
-    __device__ static void sym_pair_t(const GeoParams &, const SymTgt &t, float gi, const Src &s,
-                                      A &a, SymR &r) {
-        const float dx = (float)(t.x - s.x), dy = (float)(t.y - s.y);
+    __device__ static void sym_rest(const GeoParams &, const SymTgt &t, float gi, const Src &s,
+                                    float dx, float dy, A &a, SymR &r) {
         const float inv = rcp32(__builtin_fmaf(dx, dx, dy * dy));
         const float c = s.g * inv;
         a.u32 = __builtin_fmaf(-c, dy, a.u32);
@@ -918,6 +931,7 @@ struct PerMixed {
     static constexpr bool kNearTiles = false;
     static constexpr bool kSymNear = false;
     static constexpr int kRingUnroll = 1;
+    static constexpr bool kSymTwoPhase = false;
     __device__ static Src src(const GeoParams &p, double x, double y, double g) {
         return {x, y, (float)(p.scale * g), 0.f, 0.f, 0.f};
     }
@@ -927,10 +941,20 @@ struct PerMixed {
         a.u.zero();
         a.v.zero();
     }
+    // The half-angles X/2, Y/2 of the pair: remainder and scaling in FP64, narrowed to FP32.
+    __device__ static void half_angles(const GeoParams &p, const Tgt &t, const Src &s, float &hx,
+                                       float &hy) {
+        hx = (float)(p.hwave * remainder_fast(t.x - s.x, p.L, p.invL));
+        hy = (float)(p.hwave * remainder_fast(t.y - s.y, p.L, p.invL));
+    }
     __device__ static void terms(const GeoParams &p, const Tgt &t, const Src &s, float &fu,
                                  float &fv, bool self) {
-        const float hx = (float)(p.hwave * remainder_fast(t.x - s.x, p.L, p.invL));
-        const float hy = (float)(p.hwave * remainder_fast(t.y - s.y, p.L, p.invL));
+        float hx, hy;
+        half_angles(p, t, s, hx, hy);
+        terms_h(p, hx, hy, fu, fv, self);
+    }
+    __device__ static void terms_h(const GeoParams &p, float hx, float hy, float &fu, float &fv,
+                                   bool self) {
         const float hx2 = hx * hx, hy2 = hy * hy;
         float shx, shy, snx, sny;
         sinh_sin2_f32(hx, hx2, shx, snx);
@@ -993,11 +1017,16 @@ struct PerMixed {
     };
     __device__ static float sym_g(const GeoParams &p, double g) { return (float)(p.scale * g); }
     __device__ static SymTgt sym_tgt(const GeoParams &p, double x, double y) { return tgt(p, x, y); }
-    template <class A>
-    __device__ static void sym_pair(const GeoParams &p, const SymTgt &t, float gi, const Src &s,
-                                    A &a, SymR &r) {
+    // The pair in two parts as in InfMixed: sym_diff gives the half-angles.
+    __device__ static void sym_diff(const GeoParams &p, const SymTgt &t, const Src &s, float &hx,
+                                    float &hy) {
+        half_angles(p, t, s, hx, hy);
+    }
+    template <bool kDirect, class A>
+    __device__ static void sym_rest(const GeoParams &p, const SymTgt &, float gi, const Src &s,
+                                    float hx, float hy, A &a, SymR &r) {
         float fu, fv;
-        terms(p, t, s, fu, fv, false);
+        terms_h(p, hx, hy, fu, fv, false);
         a.u32 = __builtin_fmaf(-s.gs, fu, a.u32);
         a.v32 = __builtin_fmaf(s.gs, fv, a.v32);
         r.u = __builtin_fmaf(gi, fu, r.u);

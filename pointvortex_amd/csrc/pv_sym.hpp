@@ -76,14 +76,33 @@ __device__ __forceinline__ void sym_step(const GeoParams &gp, const typename P::
 #pragma unroll
         for (int t = 0; t < TPT; ++t)
             P::sym_apply(s, gi[t], m[t], &inv[t * ND], acc[t], r);
+    } else if constexpr (std::is_same_v<typename P::SymScalar, float>) {
+        // Mixed policies: the FP64 difference (sym_diff), then the FP32 rest (sym_rest).  With
+        // kSymTwoPhase every target's difference comes first and a scheduling barrier keeps the
+        // FP64 instructions in one group per step: FP32 instructions that follow an FP64 one in
+        // a wave's stream do not issue at the FP32 rate.  That pays on the mixed disk, whose
+        // FP32 work per pair is three reciprocal chains; on the infinite plane and the periodic
+        // box the interleaved order the compiler picks is faster.
+        float dx[TPT], dy[TPT];
+        if constexpr (P::kSymTwoPhase) {
+#pragma unroll
+            for (int t = 0; t < TPT; ++t)
+                P::sym_diff(gp, tg[t], s, dx[t], dy[t]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < TPT; ++t)
+                P::template sym_rest<kDirect>(gp, tg[t], gi[t], s, dx[t], dy[t], acc[t], r);
+        } else {
+#pragma unroll
+            for (int t = 0; t < TPT; ++t) {
+                P::sym_diff(gp, tg[t], s, dx[t], dy[t]);
+                P::template sym_rest<kDirect>(gp, tg[t], gi[t], s, dx[t], dy[t], acc[t], r);
+            }
+        }
     } else {
 #pragma unroll
-        for (int t = 0; t < TPT; ++t) {
-            if constexpr (P::kSymNear)
-                P::template sym_pair_t<kDirect>(gp, tg[t], gi[t], s, acc[t], r);
-            else
-                P::sym_pair(gp, tg[t], gi[t], s, acc[t], r);
-        }
+        for (int t = 0; t < TPT; ++t)
+            P::sym_pair(gp, tg[t], gi[t], s, acc[t], r);
     }
 }
 
