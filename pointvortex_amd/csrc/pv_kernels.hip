@@ -207,15 +207,16 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(PairLaunch a, GeoParams gp
 
 // The infinite-plane policies, whose pair is short enough that the ring's data movement is a
 // sizeable share of a step: their symmetric kernels also have 8 targets per thread.  The FP64
-// infinite and disk rings read their sources from a doubled window (win), one index add per
-// step instead of three; the infinite FP64 diagonal blocks also share one reciprocal over two
-// sources x TPT targets, as the split kernel does.  (In the mixed kernels the window costs
-// more than it saves: the compiler then loads each source in the step that uses it instead of
-// one step ahead.)
+// infinite and the disk rings read their sources from a doubled window (win), one index add
+// per step instead of three; the infinite FP64 diagonal blocks also share one reciprocal over
+// two sources x TPT targets, as the split kernel does.  (In the mixed infinite kernel the
+// window costs more than it saves: the compiler then loads each source in the step that uses
+// it instead of one step ahead.  The mixed disk's two-phase step keeps the load ahead.)
 template <class P>
 constexpr bool kInfPlane = std::is_same_v<P, InfF64> || std::is_same_v<P, InfMixed>;
 template <class P>
-constexpr bool kRingWindow = std::is_same_v<P, InfF64> || std::is_same_v<P, DiskF64>;
+constexpr bool kRingWindow =
+    std::is_same_v<P, InfF64> || std::is_same_v<P, DiskF64> || std::is_same_v<P, DiskMixed>;
 // The diagonal blocks' shared reciprocal over two sources (infinite FP64 only).
 template <class P> constexpr bool kDiagShared = std::is_same_v<P, InfF64>;
 
