@@ -432,6 +432,33 @@ def test_ensemble_disk_near_centre_source(oracle, mode):
         assert normwise(u, v, ur, vr) <= fp64_gate("disk", p.x, p.y, p.circulation, ur, vr)
 
 
+@pytest.mark.parametrize("pos", [(2e-160, 1e-161), (-3e-31, 2e-31), (6e-13, -4e-13),
+                                 (8e-13, 5e-13), (3e-7, 1e-7)])
+def test_ensemble_mixed_disk_near_centre_source(oracle, pos):
+    """Mixed disk items share one FP32 reciprocal between the two image terms (w = 1/|x|^2
+    staged with each source) unless the item holds a vortex within 2^-40 R of the centre,
+    which takes the direct form.  One such vortex per instance, in a middle tile (a source
+    for earlier tiles and a target for later ones), on both sides of that bound, at 64-, 128-
+    and 256-vortex tiles; every other instance of the launch is away from the centre."""
+    pops = [wl.disk(n, 74 + k, wall_fraction=0.0) for k, n in enumerate((700, 1100, 1800))]
+    for p in pops:
+        p.x[p.n // 2], p.y[p.n // 2] = pos
+    cfgs = [InstanceConfig("disk")] * 3
+    for tpl in (1, 2, 4):
+        e = Ensemble(sum(p.n for p in pops), 4, "mixed")
+        e.set_launch_table([("disk", "mixed", 2**64 - 1, tpl, 4)])
+        e.load(cfgs, [p.x for p in pops], [p.y for p in pops], [p.circulation for p in pops])
+        assert e.symmetric
+        us, vs = e.evaluate()
+        e.close()
+        for c, p, u, v in zip(cfgs, pops, us, vs):
+            assert np.all(np.isfinite(u)) and np.all(np.isfinite(v))
+            ur, vr = oracle.evaluate(ospec(c), p.x, p.y, p.circulation)
+            assert normwise(u, v, ur, vr) <= 1e-6, (tpl, p.n)
+            i = p.n // 2
+            assert np.hypot(u[i] - ur[i], v[i] - vr[i]) <= 1e-5 * np.hypot(ur[i], vr[i])
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_symmetric_and_ordered_ensembles_agree(precision):
     """The two work decompositions of one ensemble (instances up to 5 tiles of 128): FP64
