@@ -312,7 +312,7 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
     double xi[TPT], yi[TPT];
     typename P::SymScalar gi[TPT];
     int ti[TPT];
-    bool near = false, sym_near = false;
+    bool near = false;
 #pragma unroll
     for (int t = 0; t < TPT; ++t) {
         const int i = t0 + t * 64 + lane;
@@ -324,24 +324,18 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
         ti[t] = valid ? i : -1;
         if constexpr (P::kNearTiles)
             near |= valid && P::is_near(gp, xi[t], yi[t]);
-        if constexpr (P::kSymNear)
-            sym_near |= valid && P::is_sym_near(gp, xi[t], yi[t]);
     }
     wave_sync();
     for (int k = lane; k < span; k += 64) {
         const int j = off + s0 + (k < cnt ? k : 0);
         const double xj = a.x[j], yj = a.y[j];
-        sl->src[k] = ring_src<P>(gp, xj, yj, k < cnt ? a.g[j] : 0.0);
+        sl->src[k] = P::src(gp, xj, yj, k < cnt ? a.g[j] : 0.0);
         if constexpr (P::kNearTiles)
             near |= k < cnt && P::is_near(gp, xj, yj);
-        if constexpr (P::kSymNear)
-            sym_near |= k < cnt && P::is_sym_near(gp, xj, yj);
     }
     wave_sync();
     if constexpr (P::kNearTiles)
         near = __any(near);
-    if constexpr (P::kSymNear)
-        sym_near = __any(sym_near);
 
 
     Acc acc[TPT];
@@ -416,10 +410,10 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
                 tg[t] = P::sym_tgt(gp, xi[t], yi[t]);
             // The ring of the single-system symmetric kernel: 64 steps per group of 64
             // sources bring every source's reaction back to its first lane.  kSymNear policies
-            // take their direct form where the item holds a vortex near the centre (sym_near,
-            // uniform over the wave), as the single-system kernel does per block.
-            const auto ring = [&](auto direct) {
-                constexpr bool kDirect = decltype(direct)::value;
+            // keep their direct form here (their records come from src(), without sym_src's
+            // w; see DiskMixed::sym_src).
+            {
+                constexpr bool kDirect = P::kSymNear;
                 for (int g0 = 0; g0 < span; g0 += 64) {
                     typename P::SymR r;
                     r.u = 0;
@@ -443,11 +437,7 @@ __device__ void batch_sym_item(const BatchLaunch &a, const Instance *ins, const 
                             P::fold(acc[t]);
                     }
                 }
-            };
-            if (P::kSymNear && sym_near)
-                ring(Flag<P::kSymNear>{});
-            else
-                ring(Flag<false>{});
+            }
         }
     }
 #pragma unroll
