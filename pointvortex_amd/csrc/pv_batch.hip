@@ -528,15 +528,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_sym_reduce_kernel(BatchLaun
         fail(kFailStateNonFinite);
     else if (ins.gp.geometry == 2 && xt * xt + yt * yt >= ins.gp.R2)
         fail(kFailOutsideDisk);
-    const size_t nn = (size_t)ins.n;
-    const double *p = a.part + ins.poff + (v - ins.offset);
     Comp su, sv;
-    su.zero();
-    sv.zero();
-    for (int s = 0; s < ins.nt; ++s) {
-        su.add(__builtin_nontemporal_load(p + (size_t)(2 * s) * nn));
-        sv.add(__builtin_nontemporal_load(p + (size_t)(2 * s + 1) * nn));
-    }
+    sum_rows(a.part + ins.poff + (v - ins.offset), (size_t)ins.n, ins.nt, su, sv);
     double u = su.value(), w = sv.value();
     if (ins.gp.geometry == 2 && ins.precision == 1)
         disk_self_image(ins.gp, xt, yt, a.g[v], u, w);

@@ -516,14 +516,8 @@ __global__ __launch_bounds__(kBlock) void sym_reduce_kernel(SymLaunch a, GeoPara
             record_failure(a.st, a.epoch, kFailStateNonFinite);
         else if (gp.geometry == 2 && x * x + y * y >= gp.R2)
             record_failure(a.st, a.epoch, kFailOutsideDisk);
-        const size_t n = (size_t)a.n;
         Comp su, sv;
-        su.zero();
-        sv.zero();
-        for (int s = 0; s < a.nt; ++s) {
-            su.add(__builtin_nontemporal_load(&a.part[(size_t)(2 * s) * n + i]));
-            sv.add(__builtin_nontemporal_load(&a.part[(size_t)(2 * s + 1) * n + i]));
-        }
+        sum_rows(a.part + i, (size_t)a.n, a.nt, su, sv);
         double u = su.value(), v = sv.value();
         if (a.self_image)
             disk_self_image(gp, x, y, a.g[i], u, v);

@@ -106,4 +106,34 @@ __device__ __forceinline__ void sym_step(const GeoParams &gp, const typename P::
     }
 }
 
+// A target's velocity from its partial sums: rows s = 0..nt-1 of (u, v) at p + 2 s stride and
+// p + (2 s + 1) stride, added in row order with TwoSum.  The reductions are bound by HBM
+// latency (one load pair per row and one target per thread), so the loads of kRowBatch rows are
+// issued before their additions; the order of the additions, and so the bits, are those of the
+// plain loop.
+constexpr int kRowBatch = 8;
+__device__ __forceinline__ void sum_rows(const double *p, size_t stride, int nt, Comp &su,
+                                         Comp &sv) {
+    su.zero();
+    sv.zero();
+    int s = 0;
+    for (; s + kRowBatch <= nt; s += kRowBatch) {
+        double ru[kRowBatch], rv[kRowBatch];
+#pragma unroll
+        for (int q = 0; q < kRowBatch; ++q) {
+            ru[q] = __builtin_nontemporal_load(p + (size_t)(2 * (s + q)) * stride);
+            rv[q] = __builtin_nontemporal_load(p + (size_t)(2 * (s + q) + 1) * stride);
+        }
+#pragma unroll
+        for (int q = 0; q < kRowBatch; ++q) {
+            su.add(ru[q]);
+            sv.add(rv[q]);
+        }
+    }
+    for (; s < nt; ++s) {
+        su.add(__builtin_nontemporal_load(p + (size_t)(2 * s) * stride));
+        sv.add(__builtin_nontemporal_load(p + (size_t)(2 * s + 1) * stride));
+    }
+}
+
 } // namespace pv
