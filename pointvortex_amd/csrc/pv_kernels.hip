@@ -219,6 +219,11 @@ constexpr bool kRingWindow =
     std::is_same_v<P, InfF64> || std::is_same_v<P, DiskF64> || std::is_same_v<P, DiskMixed>;
 // The diagonal blocks' shared reciprocal over two sources (infinite FP64 only).
 template <class P> constexpr bool kDiagShared = std::is_same_v<P, InfF64>;
+// Ring steps in pairs with one product tree over both (sym_step_pair): the FP64 rings with the
+// window, up to 4 denominators per step (the infinite plane at 1-4 targets per thread, the disk
+// at 1-2; beyond that the tree's registers cost a wave per SIMD).
+template <class P, int TPT>
+constexpr bool kPairedSteps = kRingWindow<P> && P::kSymDenoms > 0 && TPT * P::kSymDenoms <= 4;
 
 // Slot of source k of a staged chunk: with the window, each group of 64 sources is staged twice
 // in a row (group k / 64 at 128 (k / 64)), so that the ring's index needs no wrap.
@@ -447,18 +452,26 @@ __attribute__((amdgpu_waves_per_eu(TPT >= 8 ? 3 : 1))) void sym_kernel(SymLaunch
                     const int step = dir == 1 ? 1 : -1;
                     typename P::Src next = sh[win<P>(g0) + lane];
                     constexpr int kUnroll = TPT >= 8 ? 1 : P::kRingUnroll;
+                    if constexpr (kPairedSteps<P, TPT>) {
+                        for (int s = 0; s < 64; s += 2) {
+                            const typename P::Src s_a = next, s_b = w[step * (s + 1)];
+                            next = w[step * (s + 2)];    // at s = 62, the first source again
+                            sym_step_pair<P, TPT>(gp, tg, gi, s_a, s_b, acc, r);
+                        }
+                    } else {
 #pragma unroll kUnroll
-                    for (int s = 0; s < 64; ++s) {
-                        // The next step's source is read one step ahead (at s = 63 this reads
-                        // the lane's first source again, unused).
-                        const typename P::Src s_a = next;
-                        if constexpr (kRingWindow<P>)
-                            next = w[step * (s + 1)];
-                        else
-                            next = sh[g0 + ((lane + dir * (s + 1)) & 63)];
-                        sym_step<P, TPT, kDirect>(gp, tg, gi, s_a, acc, r);
-                        r.u = wave_ror1(r.u);
-                        r.v = wave_ror1(r.v);
+                        for (int s = 0; s < 64; ++s) {
+                            // The next step's source is read one step ahead (at s = 63 this
+                            // reads the lane's first source again, unused).
+                            const typename P::Src s_a = next;
+                            if constexpr (kRingWindow<P>)
+                                next = w[step * (s + 1)];
+                            else
+                                next = sh[g0 + ((lane + dir * (s + 1)) & 63)];
+                            sym_step<P, TPT, kDirect>(gp, tg, gi, s_a, acc, r);
+                            r.u = wave_ror1(r.u);
+                            r.v = wave_ror1(r.v);
+                        }
                     }
                     if (g0 + lane < cnt) {
                         red[wave][g0 + lane][0] = (double)r.u;

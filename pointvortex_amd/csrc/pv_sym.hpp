@@ -136,4 +136,40 @@ __device__ __forceinline__ void sum_rows(const double *p, size_t stride, int nt,
     }
 }
 
+// Two ring steps whose denominators share one product tree (FP64 policies with kSymDenoms):
+// both sources' TPT x ND denominators, one rcp64 for all of them, then the two steps' sums, the
+// reaction rotating between them as in two sym_step calls.  Twice the tree amortizes its
+// reciprocal, its refinement and its range check over twice the pairs.
+template <class P, int TPT>
+__device__ __forceinline__ void sym_step_pair(const GeoParams &gp, const typename P::SymTgt *tg,
+                                              const typename P::SymScalar *gi,
+                                              const typename P::Src &sa, const typename P::Src &sb,
+                                              typename P::SymAcc *acc, typename P::SymR &r) {
+    constexpr int ND = P::kSymDenoms, K = 2 * TPT * ND;
+    typename P::SymTmp ma[TPT], mb[TPT];
+    double d[K], inv[K];
+#pragma unroll
+    for (int t = 0; t < TPT; ++t)
+        P::sym_prep(gp, tg[t], sa, ma[t], &d[t * ND]);
+#pragma unroll
+    for (int t = 0; t < TPT; ++t)
+        P::sym_prep(gp, tg[t], sb, mb[t], &d[(TPT + t) * ND]);
+    const bool ok = TreeInverse<K>::run(d, inv);
+    if (tree_fallback(ok)) {
+#pragma unroll
+        for (int e = 0; e < K; ++e)
+            inv[e] = rcp64(d[e]);
+    }
+#pragma unroll
+    for (int t = 0; t < TPT; ++t)
+        P::sym_apply(sa, gi[t], ma[t], &inv[t * ND], acc[t], r);
+    r.u = wave_ror1(r.u);
+    r.v = wave_ror1(r.v);
+#pragma unroll
+    for (int t = 0; t < TPT; ++t)
+        P::sym_apply(sb, gi[t], mb[t], &inv[(TPT + t) * ND], acc[t], r);
+    r.u = wave_ror1(r.u);
+    r.v = wave_ror1(r.v);
+}
+
 } // namespace pv
