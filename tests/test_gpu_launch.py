@@ -199,6 +199,38 @@ def test_symmetric_mixed_near_centre_disk_source(oracle, pos):
     assert near <= 1e-5
 
 
+@pytest.mark.parametrize("geometry", ["infinite", "disk"])
+@pytest.mark.parametrize("tpt,tile", [(1, 128), (2, 256), (4, 128)])
+def test_symmetric_fp64_close_pair_across_tiles(oracle, geometry, tpt, tile):
+    """A close pair with its two vortices in different tiles of the symmetric FP64 path, so
+    that it is evaluated in an off-diagonal block (the paired ring steps at 1-4 denominators
+    per step, the single steps at 4 targets on the disk).  On the infinite plane the pair sits
+    at the origin, down to 1e-150 apart: its denominator takes the ring's product tree out of
+    range and the wave falls back to one reciprocal per denominator.  In the disk (the pair at
+    (0.3, 0.1), where FP64 positions resolve about 1e-16) down to 1e-15.  The pair within 1e-10
+    of the oracle per vortex, the rest within normwise 1e-12."""
+    pop, k = setup(geometry, "fp64", 1500, seed=41)
+    k.set_launch(tpt, 0, tile, 1)
+    assert k.resolved_launch(pop.n, pop.n) == (tpt, 0, tile, 1)
+    i, j = 100, 1400                      # different tiles at 256, 512 and 1024 targets
+    if geometry == "infinite":
+        x0, y0, seps = 0.0, 0.0, (1e-2, 1e-6, 1e-12, 1e-50, 1e-150)
+    else:
+        x0, y0, seps = 0.3, 0.1, (1e-2, 1e-6, 1e-12, 1e-15)
+    rest = np.ones(pop.n, dtype=bool)
+    rest[[i, j]] = False
+    for sep in seps:
+        x, y = pop.x.copy(), pop.y.copy()
+        x[i], y[i] = x0, y0
+        x[j], y[j] = x0 + 0.8 * sep, y0 + 0.6 * sep
+        vel = k.evaluate(VortexSystem(x, y, pop.circulation))
+        ur, vr = oracle.evaluate(spec(geometry), x, y, pop.circulation)
+        for v in (i, j):
+            err = np.hypot(vel.x[v] - ur[v], vel.y[v] - vr[v])
+            assert err <= 1e-10 * np.hypot(ur[v], vr[v]), (sep, v)
+        assert normwise(vel.x[rest], vel.y[rest], ur[rest], vr[rest]) <= 1e-12, sep
+
+
 def test_symmetric_path_near_centre_disk_source(oracle):
     pop, k = setup("disk", "fp64", 1800, seed=71)
     x, y = pop.x.copy(), pop.y.copy()
