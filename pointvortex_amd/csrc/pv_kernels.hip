@@ -219,6 +219,9 @@ constexpr bool kRingWindow =
     std::is_same_v<P, InfF64> || std::is_same_v<P, DiskF64> || std::is_same_v<P, DiskMixed>;
 // The diagonal blocks' shared reciprocal over two sources (infinite FP64 only).
 template <class P> constexpr bool kDiagShared = std::is_same_v<P, InfF64>;
+// The mixed infinite ring at 4 targets per thread rotates its reaction with ds_bpermute
+// (lds_ror1), the others with DPP (at 2 targets the bpermute's latency is exposed: 38 % slower).
+template <class P, int TPT> constexpr bool kLdsRotate = std::is_same_v<P, InfMixed> && TPT == 4;
 // Ring steps in pairs with one product tree over both (sym_step_pair): the FP64 rings with the
 // window, up to 4 denominators per step (the infinite plane at 1-4 targets per thread, the disk
 // at 1-2; beyond that the tree's registers cost a wave per SIMD).
@@ -391,7 +394,8 @@ __attribute__((amdgpu_waves_per_eu(TPT >= 8 ? 3 : 1))) void sym_kernel(SymLaunch
 #pragma unroll
         for (int t = 0; t < TPT; ++t)
             tg[t] = P::sym_tgt(gp, xi[t], yi[t]);
-        const int dir = ring_dir(lane);
+        const int dir = kLdsRotate<P, TPT> ? 1 : ring_dir(lane);
+        const int rot_addr = lds_ror1_addr(lane);
         for (int base = s0; base < s1; base += TILE) {
             const int cnt = min(TILE, s1 - base);
             const int span = (cnt + 63) & ~63;          // <= TILE: TILE is a multiple of 64
@@ -469,8 +473,13 @@ __attribute__((amdgpu_waves_per_eu(TPT >= 8 ? 3 : 1))) void sym_kernel(SymLaunch
                             else
                                 next = sh[g0 + ((lane + dir * (s + 1)) & 63)];
                             sym_step<P, TPT, kDirect>(gp, tg, gi, s_a, acc, r);
-                            r.u = wave_ror1(r.u);
-                            r.v = wave_ror1(r.v);
+                            if constexpr (kLdsRotate<P, TPT>) {
+                                r.u = lds_ror1(r.u, rot_addr);
+                                r.v = lds_ror1(r.v, rot_addr);
+                            } else {
+                                r.u = wave_ror1(r.u);
+                                r.v = wave_ror1(r.v);
+                            }
                         }
                     }
                     if (g0 + lane < cnt) {

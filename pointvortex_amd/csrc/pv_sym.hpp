@@ -29,6 +29,15 @@ __device__ __forceinline__ float wave_ror1(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppWaveRor1, 0xF, 0xF, true));
 }
 
+// The ring's rotation through the LDS crossbar (ds_bpermute) instead of a DPP move: lane l
+// takes lane l + 1's value, so the ring's step is +1.  It issues on the LDS pipe rather than
+// the VALU, which pays where the VALU is the bound and the step is short (the mixed infinite
+// plane).
+__device__ __forceinline__ float lds_ror1(float v, int addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ int lds_ror1_addr(int lane) { return ((lane + 1) & 63) << 2; }
+
 // A compile-time flag passed as a value (to instantiate a generic lambda both ways).
 template <bool B> struct Flag {
     static constexpr bool value = B;
